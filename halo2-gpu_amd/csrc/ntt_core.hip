@@ -1,0 +1,357 @@
+// Fr NTT engine: table generation and the LDS-staged Stockham pass kernel.
+// See ntt_core.h for the decomposition.
+#include "ntt_core.h"
+
+namespace h2 {
+
+// zeta (a primitive cube root of unity, halo2's g_coset) and zeta^2 in
+// Montgomery form; the same values the reference hard-codes at
+// cuda/include/kernel/zeta.h:19-30.
+__device__ __forceinline__ Fr zeta_pow(uint32_t m)
+{
+    Fr r;
+    if (m == 1) {
+        const uint32_t z[8] = { 0x55fcd653u, 0x0363f299u, 0x5fc1e200u, 0x73e7950bu, 0x576d9d24u, 0xc5fce83eu,
+            0xa1c3a4d4u, 0x059c805du };
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            r.v[i] = z[i];
+    } else {
+        const uint32_t z[8] = { 0x4a0329b3u, 0x93e7cedeu, 0x7a96c167u, 0x7d4fdca7u, 0xb19a750au, 0x8be4ba08u,
+            0xa5661c25u, 0x1cbd5653u };
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            r.v[i] = z[i];
+    }
+    return r;
+}
+
+NttPlan ntt_plan(uint32_t log_n)
+{
+    NttPlan p {};
+    if (log_n <= NTT_MAX_LOG_R) {
+        p.P = 1;
+        p.log_r[0] = log_n;
+    } else if (log_n <= 2 * NTT_MAX_LOG_R) {
+        p.P = 2;
+        p.log_r[0] = (log_n + 1) / 2;
+        p.log_r[1] = log_n - p.log_r[0];
+    } else {
+        p.P = 3;
+        p.log_r[0] = (log_n + 2) / 3;
+        p.log_r[1] = (log_n - p.log_r[0] + 1) / 2;
+        p.log_r[2] = log_n - p.log_r[0] - p.log_r[1];
+    }
+    for (int i = 0; i < p.P; ++i) {
+        // aim for >= 256 work-groups per transform, <= 4096 elements per work-group
+        int g = (int)log_n - (int)p.log_r[i] - 8;
+        int gmax = (int)NTT_MAX_LOG_R - (int)p.log_r[i];
+        if (g > gmax)
+            g = gmax;
+        if (g < 0)
+            g = 0;
+        if (p.P == 1)
+            g = 0;
+        p.log_g[i] = (uint32_t)g;
+    }
+    return p;
+}
+
+bool ntt_needs_scratch(uint32_t log_n, bool aliased)
+{
+    NttPlan p = ntt_plan(log_n);
+    return p.P == 3 || (p.P == 2 && aliased);
+}
+
+static inline uint32_t split_h(uint32_t log_n) { return (log_n + 1) / 2; }
+
+void ntt_tables_size(WorkspaceSizer& ws, uint32_t log_n, bool powers_twist)
+{
+    NttPlan p = ntt_plan(log_n);
+    uint32_t h = split_h(log_n);
+    ws.take(32ull << h);
+    ws.take(32ull << (log_n - h));
+    for (int i = 0; i < p.P; ++i)
+        ws.take(32ull * ((1ull << p.log_r[i]) / 2 + 1));
+    if (powers_twist) {
+        ws.take(32ull << h);
+        ws.take(32ull << (log_n - h));
+    }
+}
+
+struct TableJob {
+    Fr* out;
+    uint32_t shift;
+    uint32_t count;
+    uint32_t use_c;
+};
+struct TableJobs {
+    TableJob j[7];
+};
+
+// out[i] = (base^(2^shift))^i, base = omega (device) or c (by value)
+__global__ void ntt_tables_kernel(TableJobs tj, const Fr* d_omega, Fr c)
+{
+    const TableJob job = tj.j[blockIdx.y];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= job.count)
+        return;
+    Fr b = job.use_c ? c : Fr::load(d_omega);
+    for (uint32_t s = 0; s < job.shift; ++s)
+        b = Fr::sqr(b);
+    Fr::pow_u64(b, i).store(job.out + i);
+}
+
+RustError ntt_tables_build(Workspace& ws, NttTables& t, uint32_t log_n, const Fr* d_omega, bool powers_twist,
+    const Fr& c, hipStream_t stream)
+{
+    NttPlan p = ntt_plan(log_n);
+    uint32_t h = split_h(log_n);
+    TableJobs tj {};
+    int nj = 0;
+    uint32_t maxc = 0;
+    auto add = [&](uint64_t count, uint32_t shift, uint32_t use_c) -> Fr* {
+        Fr* out = (Fr*)ws.take(32ull * count);
+        tj.j[nj++] = TableJob { out, shift, (uint32_t)count, use_c };
+        if (count > maxc)
+            maxc = (uint32_t)count;
+        return out;
+    };
+    t.h = h;
+    t.tw_lo = add(1ull << h, 0, 0);
+    t.tw_hi = add(1ull << (log_n - h), h, 0);
+    for (int i = 0; i < p.P; ++i)
+        t.twr[i] = add((1ull << p.log_r[i]) / 2 + 1, log_n - p.log_r[i], 0);
+    if (powers_twist) {
+        t.c_lo = add(1ull << h, 0, 1);
+        t.c_hi = add(1ull << (log_n - h), h, 1);
+    }
+    if (ws.overflow)
+        return fail((int)hipErrorInvalidValue, "NTT workspace too small for the twiddle tables (log_n=%u)", log_n);
+    dim3 grid((maxc + 127) / 128, nj);
+    ntt_tables_kernel<<<grid, 128, 0, stream>>>(tj, d_omega, c);
+    H2_LAUNCH_CHECK();
+    return ok();
+}
+
+struct PassBatch {
+    const Fr* in[NTT_MAX_BATCH];
+    Fr* out[NTT_MAX_BATCH];
+};
+
+struct PassParams {
+    uint32_t log_n, log_r, log_g, log_l;
+    uint64_t in_len;
+    const Fr* tw_lo;
+    const Fr* tw_hi;
+    const Fr* twr;
+    const Fr* c_lo;
+    const Fr* c_hi;
+    uint32_t h;
+    int twist;
+    int post;
+    int last;
+    uint32_t xcd_remap;
+    Fr scale;
+};
+
+__device__ __forceinline__ Fr lds_get(const uint4* lds, uint32_t i)
+{
+    uint4 a = lds[2 * i], b = lds[2 * i + 1];
+    Fr r;
+    r.v[0] = a.x; r.v[1] = a.y; r.v[2] = a.z; r.v[3] = a.w;
+    r.v[4] = b.x; r.v[5] = b.y; r.v[6] = b.z; r.v[7] = b.w;
+    return r;
+}
+__device__ __forceinline__ void lds_put(uint4* lds, uint32_t i, const Fr& x)
+{
+    lds[2 * i] = make_uint4(x.v[0], x.v[1], x.v[2], x.v[3]);
+    lds[2 * i + 1] = make_uint4(x.v[4], x.v[5], x.v[6], x.v[7]);
+}
+
+__global__ void __launch_bounds__(1024) ntt_pass_kernel(PassBatch pb, PassParams pp)
+{
+    extern __shared__ uint4 lds[];
+    const uint32_t log_r = pp.log_r, log_g = pp.log_g, log_l = pp.log_l;
+    const uint32_t R = 1u << log_r;
+    const uint32_t G = 1u << log_g;
+    const uint32_t RG = R << log_g;
+    const uint32_t T = blockDim.x;
+    const uint32_t nwg = (uint32_t)((1ull << pp.log_n) >> (log_r + log_g));
+    uint32_t b = blockIdx.x;
+    if (pp.xcd_remap) // blocks are dealt to the 8 XCDs round robin: keep neighbouring columns on one L2
+        b = (b & 7u) * (nwg >> 3) + (b >> 3);
+    const uint32_t j0 = b << log_g;
+    const Fr* __restrict__ in = pb.in[blockIdx.y];
+    Fr* __restrict__ out = pb.out[blockIdx.y];
+    const uint64_t stride = (1ull << pp.log_n) >> log_r;
+
+    for (uint32_t f = threadIdx.x; f < RG; f += T) {
+        const uint32_t g = f & (G - 1);
+        const uint32_t n1 = f >> log_g;
+        const uint64_t gi = (uint64_t)(j0 + g) + (uint64_t)n1 * stride;
+        Fr x;
+        if (gi < pp.in_len) {
+            x = Fr::load(in + gi);
+            if (pp.twist == TWIST_ZETA) {
+                uint32_t m = (uint32_t)(gi % 3u);
+                if (m)
+                    x = x * zeta_pow(m);
+            } else if (pp.twist == TWIST_POWERS) {
+                const uint32_t hm = (1u << pp.h) - 1;
+                const uint32_t lo = (uint32_t)gi & hm, hi = (uint32_t)(gi >> pp.h);
+                Fr w = Fr::load(pp.c_lo + lo);
+                if (hi)
+                    w = w * Fr::load(pp.c_hi + hi);
+                x = x * w;
+            }
+        } else {
+            x = Fr::zero();
+        }
+        const uint32_t slot = (g << log_r) + (__brev(n1) >> (32 - log_r));
+        lds_put(lds, slot, x);
+    }
+
+    const uint32_t halfR = R >> 1;
+    for (uint32_t s = 0; s < log_r; ++s) {
+        __syncthreads();
+        const uint32_t half = 1u << s;
+        for (uint32_t q = threadIdx.x; q < (RG >> 1); q += T) {
+            const uint32_t g = q >> (log_r - 1);
+            const uint32_t qq = q & (halfR - 1);
+            const uint32_t pos = qq & (half - 1);
+            const uint32_t blk = qq >> s;
+            const uint32_t i0 = (g << log_r) + (blk << (s + 1)) + pos;
+            const uint32_t i1 = i0 + half;
+            Fr a = lds_get(lds, i0);
+            Fr c = lds_get(lds, i1);
+            if (pos)
+                c = c * Fr::load(pp.twr + (pos << (log_r - 1 - s)));
+            lds_put(lds, i0, a + c);
+            lds_put(lds, i1, a - c);
+        }
+    }
+    __syncthreads();
+
+    const bool mapA = (G > 1) && (log_l > 0);
+    const uint32_t lmask = (1u << log_l) - 1;
+    for (uint32_t f = threadIdx.x; f < RG; f += T) {
+        uint32_t g, k1;
+        if (mapA) {
+            g = f & (G - 1);
+            k1 = f >> log_g;
+        } else {
+            k1 = f & (R - 1);
+            g = f >> log_r;
+        }
+        Fr x = lds_get(lds, (g << log_r) + k1);
+        const uint32_t j = j0 + g;
+        const uint32_t K = j & lmask;
+        const uint32_t n2 = j >> log_l;
+        const uint64_t o = ((uint64_t)n2 << (log_l + log_r)) + K + ((uint64_t)k1 << log_l);
+        if (!pp.last) {
+            const uint32_t e = (n2 * k1) << log_l;
+            if (e) {
+                const uint32_t hm = (1u << pp.h) - 1;
+                Fr w = Fr::load(pp.tw_lo + (e & hm));
+                const uint32_t hi = e >> pp.h;
+                if (hi)
+                    w = w * Fr::load(pp.tw_hi + hi);
+                x = x * w;
+            }
+        } else if (pp.post != POST_NONE) {
+            x = x * pp.scale;
+            if (pp.post == POST_SCALE_ZETA_INV) {
+                uint32_t m = (uint32_t)(o % 3u);
+                if (m)
+                    x = x * zeta_pow(3u - m);
+            }
+        }
+        x.store(out + o);
+    }
+}
+
+static bool g_attr_set = false;
+
+RustError ntt_run(const NttTables& t, uint32_t log_n, const NttJob* jobs, int count, uint64_t in_len,
+    Fr* scratch, uint64_t scratch_slots, NttTwist twist, NttPost post, const Fr& scale, hipStream_t stream)
+{
+    if (count <= 0)
+        return ok();
+    H2_CHECK_ARG(log_n >= 1 && log_n <= NTT_MAX_LOG, "NTT log_n=%u out of range [1, %u]", log_n, NTT_MAX_LOG);
+    if (!g_attr_set) {
+        H2_TRY(hipFuncSetAttribute((const void*)ntt_pass_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+            32 << NTT_MAX_LOG_R));
+        g_attr_set = true;
+    }
+    const NttPlan p = ntt_plan(log_n);
+    const uint64_t N = 1ull << log_n;
+    bool aliased = false;
+    for (int i = 0; i < count; ++i)
+        aliased |= (const void*)jobs[i].in == (const void*)jobs[i].out;
+    const bool need_scratch = p.P == 3 || (p.P == 2 && aliased);
+    H2_CHECK_ARG(!need_scratch || (scratch != nullptr && scratch_slots > 0), "NTT of 2^%u needs a scratch buffer",
+        log_n);
+    int max_batch = NTT_MAX_BATCH;
+    if (need_scratch && scratch_slots < (uint64_t)max_batch)
+        max_batch = (int)scratch_slots;
+
+    for (int base = 0; base < count; base += max_batch) {
+        const int nb = count - base < max_batch ? count - base : max_batch;
+        for (int pass = 0; pass < p.P; ++pass) {
+            PassBatch pb {};
+            for (int i = 0; i < nb; ++i) {
+                const NttJob& jb = jobs[base + i];
+                Fr* s = scratch ? scratch + (uint64_t)i * N : nullptr;
+                const Fr* src;
+                Fr* dst;
+                if (p.P == 1) {
+                    src = jb.in;
+                    dst = jb.out;
+                } else if (p.P == 2) {
+                    Fr* mid = need_scratch ? s : jb.out;
+                    src = pass == 0 ? jb.in : mid;
+                    dst = pass == 0 ? mid : jb.out;
+                } else {
+                    src = pass == 0 ? jb.in : (pass == 1 ? s : jb.out);
+                    dst = pass == 0 ? s : jb.out;
+                }
+                pb.in[i] = src;
+                pb.out[i] = dst;
+            }
+            PassParams pp {};
+            pp.log_n = log_n;
+            pp.log_r = p.log_r[pass];
+            pp.log_g = p.log_g[pass];
+            uint32_t log_l = 0;
+            for (int q = 0; q < pass; ++q)
+                log_l += p.log_r[q];
+            pp.log_l = log_l;
+            pp.in_len = pass == 0 ? in_len : N;
+            pp.tw_lo = t.tw_lo;
+            pp.tw_hi = t.tw_hi;
+            pp.h = t.h;
+            pp.twr = t.twr[pass];
+            pp.c_lo = t.c_lo;
+            pp.c_hi = t.c_hi;
+            pp.twist = pass == 0 ? (int)twist : TWIST_NONE;
+            pp.last = pass == p.P - 1;
+            pp.post = pp.last ? (int)post : POST_NONE;
+            pp.scale = scale;
+            const uint32_t RG = 1u << (pp.log_r + pp.log_g);
+            const uint32_t nwg = (uint32_t)(N >> (pp.log_r + pp.log_g));
+            pp.xcd_remap = (nwg % 8 == 0 && nwg >= 64) ? 1u : 0u;
+            uint32_t T = RG / 2;
+            if (T > 1024)
+                T = 1024;
+            if (T < 64)
+                T = 64;
+            dim3 grid(nwg, nb);
+            ntt_pass_kernel<<<grid, T, 32u * RG, stream>>>(pb, pp);
+            H2_LAUNCH_CHECK();
+        }
+    }
+    return ok();
+}
+
+} // namespace h2

@@ -1,0 +1,503 @@
+// Batch inversion, the multiplicative prefix scan (grand product) and the
+// permutation / lookup product numerators and denominators, mirroring
+// cuda/src/polynomial/operations.cu, cuda/src/commitment/grand_product.cu,
+// permutation_commit.cu and commit_product.cu of the reference.
+//
+// Batch inversion: Montgomery's trick at two levels. Each thread folds 8
+// elements (zeros are skipped and stay zero, ff::BatchInvert semantics), the
+// 256 thread products of a work-group are combined with LDS prefix/suffix
+// scans, and one field inversion per 2048 elements finishes the work-group.
+//
+// Grand product: in-place multiplicative scan. A level scans chunks of 2048
+// elements locally; the last element of every chunk (the chunk product) forms
+// a strided view that is scanned recursively in place, and a fix-up pass
+// multiplies each chunk by the scanned product of the chunks before it. No
+// scratch memory is needed, matching the scratch-free device entry point.
+#include "common.h"
+#include "field.h"
+
+using namespace h2;
+
+namespace {
+
+constexpr unsigned T = 256; // threads per work-group
+constexpr unsigned E = 8; // elements per thread
+constexpr unsigned C = T * E; // elements per work-group
+
+// inclusive Hillis-Steele product scan over the T values in s (in place)
+template <class F>
+__device__ void block_scan_inclusive(F* s, unsigned tid, bool reverse)
+{
+    for (unsigned d = 1; d < T; d <<= 1) {
+        __syncthreads();
+        F v;
+        bool take;
+        if (!reverse) {
+            take = tid >= d;
+            if (take)
+                v = s[tid - d];
+        } else {
+            take = tid + d < T;
+            if (take)
+                v = s[tid + d];
+        }
+        __syncthreads();
+        if (take)
+            s[tid] = F::mul(v, s[tid]);
+    }
+    __syncthreads();
+}
+
+template <class F>
+__global__ void __launch_bounds__(T) batch_invert_kernel(F* __restrict__ data, uint64_t n)
+{
+    __shared__ F pre[T];
+    __shared__ F suf[T];
+    __shared__ F inv_total;
+    const unsigned tid = threadIdx.x;
+    const uint64_t base = (uint64_t)blockIdx.x * C;
+    F part[E];
+    F acc = F::one();
+#pragma unroll
+    for (unsigned k = 0; k < E; ++k) {
+        const uint64_t i = base + (uint64_t)k * T + tid;
+        part[k] = acc;
+        if (i < n) {
+            F a = F::load(data + i);
+            if (!a.is_zero())
+                acc = F::mul(acc, a);
+        }
+    }
+    pre[tid] = acc;
+    suf[tid] = acc;
+    block_scan_inclusive(pre, tid, false);
+    block_scan_inclusive(suf, tid, true);
+    if (tid == 0)
+        inv_total = F::inverse(pre[T - 1]);
+    __syncthreads();
+    // inverse of this thread's product = (prod of others before) * (after) / total
+    F inv = inv_total;
+    if (tid > 0)
+        inv = F::mul(inv, pre[tid - 1]);
+    if (tid + 1 < T)
+        inv = F::mul(inv, suf[tid + 1]);
+#pragma unroll
+    for (int k = (int)E - 1; k >= 0; --k) {
+        const uint64_t i = base + (uint64_t)k * T + tid;
+        if (i < n) {
+            F a = F::load(data + i);
+            if (!a.is_zero()) {
+                F::mul(inv, part[k]).store(data + i);
+                inv = F::mul(inv, a);
+            }
+        }
+    }
+}
+
+template <class F>
+RustError batch_invert(F* data, uint64_t n, hipStream_t stream)
+{
+    if (n == 0)
+        return ok();
+    const uint64_t blocks = (n + C - 1) / C;
+    H2_CHECK_ARG(blocks < (1ull << 31), "batch invert of %lu elements is too large", (unsigned long)n);
+    batch_invert_kernel<F><<<(unsigned)blocks, T, 0, stream>>>(data, n);
+    H2_LAUNCH_CHECK();
+    return ok();
+}
+
+// ------------------------------------------------------------------- scan
+// Strided view A[t] = base[t * s], t < m.
+
+// Local inclusive scan of each chunk of C view elements. With `prefix` non-null
+// (single-chunk top level) every element is also multiplied by *prefix.
+__global__ void __launch_bounds__(T) scan_local_kernel(Fr* __restrict__ base, uint64_t s, uint64_t m,
+    const Fr* __restrict__ prefix)
+{
+    __shared__ Fr tot[T];
+    const unsigned tid = threadIdx.x;
+    const uint64_t c0 = (uint64_t)blockIdx.x * C + (uint64_t)tid * E;
+    Fr v[E];
+    Fr acc = Fr::one();
+#pragma unroll
+    for (unsigned k = 0; k < E; ++k) {
+        const uint64_t t = c0 + k;
+        if (t < m) {
+            acc = Fr::mul(acc, Fr::load(base + t * s));
+        }
+        v[k] = acc;
+    }
+    tot[tid] = acc;
+    block_scan_inclusive(tot, tid, false);
+    Fr p = tid > 0 ? tot[tid - 1] : Fr::one();
+    if (prefix)
+        p = Fr::mul(p, Fr::load(prefix));
+    const bool apply = tid > 0 || prefix != nullptr;
+#pragma unroll
+    for (unsigned k = 0; k < E; ++k) {
+        const uint64_t t = c0 + k;
+        if (t < m)
+            (apply ? Fr::mul(v[k], p) : v[k]).store(base + t * s);
+    }
+}
+
+// Chunk u >= 1 is multiplied by the scanned chunk product A[u*C - 1]; chunk 0
+// by *prefix. The last element of every non-final chunk is already final.
+__global__ void __launch_bounds__(T) scan_fixup_kernel(Fr* __restrict__ base, uint64_t s, uint64_t m,
+    const Fr* __restrict__ prefix)
+{
+    const uint64_t u = blockIdx.x;
+    const uint64_t nchunks = (m + C - 1) / C;
+    const Fr f = u == 0 ? Fr::load(prefix) : Fr::load(base + (u * C - 1) * s);
+    const uint64_t first = u * C;
+    uint64_t last = first + C; // exclusive
+    if (u + 1 < nchunks)
+        last -= 1; // skip the chunk product, scanned at the level above
+    if (last > m)
+        last = m;
+    for (uint64_t t = first + threadIdx.x; t < last; t += T)
+        Fr::mul(Fr::load(base + t * s), f).store(base + t * s);
+}
+
+RustError scan_view(Fr* base, uint64_t s, uint64_t m, const Fr* prefix, hipStream_t stream)
+{
+    if (m == 0)
+        return ok();
+    if (m <= C) {
+        scan_local_kernel<<<1, T, 0, stream>>>(base, s, m, prefix);
+        H2_LAUNCH_CHECK();
+        return ok();
+    }
+    const uint64_t nchunks = (m + C - 1) / C;
+    scan_local_kernel<<<(unsigned)nchunks, T, 0, stream>>>(base, s, m, nullptr);
+    H2_LAUNCH_CHECK();
+    // chunk products of chunks 0..nchunks-2 sit at view positions u*C + C - 1
+    RustError e = scan_view(base + (C - 1) * s, s * C, nchunks - 1, prefix, stream);
+    if (e.code)
+        return e;
+    scan_fixup_kernel<<<(unsigned)nchunks, T, 0, stream>>>(base, s, m, prefix);
+    H2_LAUNCH_CHECK();
+    return ok();
+}
+
+// ------------------------------------------------- permutation / lookup
+
+constexpr unsigned kEwBlock = 256;
+unsigned ew_grid(uint64_t n) { return grid_for(n, kEwBlock, 256u * 16u); }
+
+// den[i] *= beta * perm[i] + gamma + value[i]        (permutation_commit.h:12-38)
+__global__ void perm_denominator_kernel(Fr* __restrict__ den, const Fr* __restrict__ perm, const Fr* __restrict__ val,
+    const Fr* __restrict__ d_beta, const Fr* __restrict__ d_gamma, uint64_t n)
+{
+    const Fr beta = Fr::load(d_beta), gamma = Fr::load(d_gamma);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        Fr t = beta * Fr::load(perm + i) + gamma + Fr::load(val + i);
+        (Fr::load(den + i) * t).store(den + i);
+    }
+}
+
+// num[i] (*)= deltaomega * omega^i * beta + gamma + value[i]   (permutation_commit.h:40-75)
+// `first` initialises num instead of multiplying into it.
+__global__ void perm_numerator_kernel(Fr* __restrict__ num, const Fr* __restrict__ val, const Fr* __restrict__ d_beta,
+    const Fr* __restrict__ d_gamma, const Fr* __restrict__ d_omega, const Fr* __restrict__ d_deltaomega, uint64_t n,
+    int first)
+{
+    const Fr beta = Fr::load(d_beta), gamma = Fr::load(d_gamma), w = Fr::load(d_omega);
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    if (tid >= n)
+        return;
+    Fr dw = Fr::load(d_deltaomega) * Fr::pow_u64(w, tid);
+    const Fr wstep = Fr::pow_u64(w, stride);
+    for (uint64_t i = tid; i < n; i += stride) {
+        Fr t = dw * beta + gamma + Fr::load(val + i);
+        if (first)
+            t.store(num + i);
+        else
+            (Fr::load(num + i) * t).store(num + i);
+        dw = dw * wstep;
+    }
+}
+
+__global__ void mul_into_kernel(Fr* __restrict__ a, const Fr* __restrict__ b, uint64_t n)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        (Fr::load(a + i) * Fr::load(b + i)).store(a + i);
+}
+
+__global__ void scalar_mul_kernel(Fr* __restrict__ a, const Fr* __restrict__ b)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        (Fr::load(a) * Fr::load(b)).store(a);
+}
+
+// (A' + beta)(S' + gamma)                             (commit_product.h:12-38)
+__global__ void lookup_denominator_kernel(Fr* __restrict__ out, const Fr* __restrict__ pi, const Fr* __restrict__ pt,
+    const Fr* __restrict__ d_beta, const Fr* __restrict__ d_gamma, uint64_t n)
+{
+    const Fr beta = Fr::load(d_beta), gamma = Fr::load(d_gamma);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        ((Fr::load(pi + i) + beta) * (Fr::load(pt + i) + gamma)).store(out + i);
+}
+
+// out *= (A + beta)(S + gamma)                        (commit_product.h:40-66)
+__global__ void lookup_numerator_kernel(Fr* __restrict__ out, const Fr* __restrict__ ci, const Fr* __restrict__ ct,
+    const Fr* __restrict__ d_beta, const Fr* __restrict__ d_gamma, uint64_t n)
+{
+    const Fr beta = Fr::load(d_beta), gamma = Fr::load(d_gamma);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        Fr v = Fr::load(out + i) * (Fr::load(ci + i) + beta);
+        (v * (Fr::load(ct + i) + gamma)).store(out + i);
+    }
+}
+
+// Folds `batch` columns into the denominator and numerator accumulators; the
+// first column of a call with `init_num` initialises the numerator.
+RustError permutation_product_columns(Fr* den, Fr* num, const FFITraitObject* perms, const FFITraitObject* values,
+    const Fr* d_beta, const Fr* d_gamma, const Fr* d_delta, const Fr* d_omega, Fr* d_deltaomega, uint64_t n,
+    uint64_t offset, uint64_t batch, bool init_num, hipStream_t stream)
+{
+    for (uint64_t c = 0; c < batch; ++c) {
+        const Fr* perm = (const Fr*)perms[c].ptr + offset;
+        const Fr* val = (const Fr*)values[c].ptr + offset;
+        perm_denominator_kernel<<<ew_grid(n), kEwBlock, 0, stream>>>(den, perm, val, d_beta, d_gamma, n);
+        H2_LAUNCH_CHECK();
+        perm_numerator_kernel<<<ew_grid(n), kEwBlock, 0, stream>>>(num, val, d_beta, d_gamma, d_omega, d_deltaomega,
+            n, init_num && c == 0);
+        H2_LAUNCH_CHECK();
+        scalar_mul_kernel<<<1, 64, 0, stream>>>(d_deltaomega, d_delta);
+        H2_LAUNCH_CHECK();
+    }
+    return ok();
+}
+
+// den <- num / den
+RustError permutation_product_finish(Fr* den, Fr* num, uint64_t n, hipStream_t stream)
+{
+    RustError e = batch_invert<Fr>(den, n, stream);
+    if (e.code)
+        return e;
+    mul_into_kernel<<<ew_grid(n), kEwBlock, 0, stream>>>(den, num, n);
+    H2_LAUNCH_CHECK();
+    return ok();
+}
+
+RustError permutation_product_device(Fr* den, Fr* num, const FFITraitObject* perms, const FFITraitObject* values,
+    const Fr* d_beta, const Fr* d_gamma, const Fr* d_delta, const Fr* d_omega, Fr* d_deltaomega, uint64_t n,
+    uint64_t offset, uint64_t batch, hipStream_t stream)
+{
+    H2_CHECK_ARG(batch > 0, "permutation product needs at least one column");
+    RustError e = permutation_product_columns(den, num, perms, values, d_beta, d_gamma, d_delta, d_omega, d_deltaomega,
+        n, offset, batch, true, stream);
+    if (e.code)
+        return e;
+    return permutation_product_finish(den, num, n, stream);
+}
+
+} // namespace
+
+extern "C" {
+
+RustError _halo2_batch_invert(void* d_data, uint32_t field_type, size_t length, void* stream)
+{
+    if (field_type != 0 && field_type != 1)
+        return fail((int)hipErrorInvalidValue, "Invalid scalar type");
+    if (length == 0)
+        return fail((int)hipErrorInvalidValue, "_halo2_batch_invert: length must be >= 1");
+    if (field_type == 0)
+        return batch_invert<Fr>((Fr*)d_data, length, (hipStream_t)stream);
+    return batch_invert<Fq>((Fq*)d_data, length, (hipStream_t)stream);
+}
+
+// ------------------------------------------------------- grand product
+
+size_t _halo2_grand_product_max_len(size_t length, uint64_t free_bytes)
+{
+    for (uint64_t l = length; l > 0; l >>= 1)
+        if (32 * l + 32 < free_bytes)
+            return l;
+    return 0;
+}
+
+uint64_t _halo2_grand_product_workspace_size(uint64_t poly_length) { return align_up(32 * poly_length + 32, 256) + 256; }
+
+RustError _halo2_grand_product(const FFITraitObject* output, const FFITraitObject* input, const FFITraitObject* prefix,
+    size_t length, size_t offset, void* scratch, uint64_t scratch_bytes, void* stream_)
+{
+    hipStream_t stream = (hipStream_t)stream_;
+    if (length == 0)
+        return ok();
+    Workspace ws(scratch, scratch_bytes);
+    Fr* d = (Fr*)ws.take(32 * length);
+    Fr* p = (Fr*)ws.take(32);
+    if (ws.overflow)
+        return fail((int)hipErrorInvalidValue, "_halo2_grand_product: workspace of %lu bytes is too small",
+            (unsigned long)scratch_bytes);
+    H2_TRY(hipMemcpyAsync(d, (const uint8_t*)input->ptr + 32 * offset, 32 * length, hipMemcpyHostToDevice, stream));
+    H2_TRY(hipMemcpyAsync(p, prefix->ptr, 32, hipMemcpyHostToDevice, stream));
+    RustError e = scan_view(d, 1, length, p, stream);
+    if (e.code)
+        return e;
+    H2_TRY(hipMemcpyAsync((uint8_t*)output->ptr + 32 * offset, d, 32 * length, hipMemcpyDeviceToHost, stream));
+    H2_TRY(hipStreamSynchronize(stream));
+    return ok();
+}
+
+RustError _halo2_grand_product_device_inputs(const FFITraitObject* d_inout, const FFITraitObject* d_prefix,
+    size_t length, size_t offset, void* stream)
+{
+    if (length == 0)
+        return ok();
+    return scan_view((Fr*)d_inout->ptr + offset, 1, length, (const Fr*)d_prefix->ptr, (hipStream_t)stream);
+}
+
+// ------------------------------------------------- permutation product
+
+size_t _halo2_permutation_product_max_len(size_t length, uint64_t free_bytes)
+{
+    for (uint64_t l = length; l > 0; l >>= 1)
+        if (4 * 32 * l + 4096 < free_bytes)
+            return l;
+    return 0;
+}
+
+uint64_t _halo2_permutation_product_workspace_size(uint64_t poly_length)
+{
+    // denominators, numerators, and one staged (permutation, value) column pair
+    return 4 * align_up(32 * poly_length, 256) + 256;
+}
+
+RustError _halo2_permutation_product(const FFITraitObject* modified_values, const FFITraitObject* permutations,
+    const FFITraitObject* values, const void* beta_device, const void* gamma_device, const void* delta_device,
+    const void* omega_device, void* deltaomega_device, size_t length, size_t offset, size_t batch, void* scratch,
+    uint64_t scratch_bytes, void* stream_)
+{
+    hipStream_t stream = (hipStream_t)stream_;
+    if (length == 0)
+        return ok();
+    Workspace ws(scratch, scratch_bytes);
+    Fr* den = (Fr*)ws.take(32 * length);
+    Fr* num = (Fr*)ws.take(32 * length);
+    Fr* perm = (Fr*)ws.take(32 * length);
+    Fr* val = (Fr*)ws.take(32 * length);
+    if (ws.overflow)
+        return fail((int)hipErrorInvalidValue, "_halo2_permutation_product: workspace of %lu bytes is too small",
+            (unsigned long)scratch_bytes);
+    H2_CHECK_ARG(batch > 0, "_halo2_permutation_product: batch must be >= 1");
+    // the running accumulator is the caller's chunk buffer itself (permutation_commit.cu:84)
+    H2_TRY(hipMemcpyAsync(den, modified_values->ptr, 32 * length, hipMemcpyHostToDevice, stream));
+    for (size_t c = 0; c < batch; ++c) {
+        H2_TRY(hipMemcpyAsync(perm, (const uint8_t*)permutations[c].ptr + 32 * offset, 32 * length,
+            hipMemcpyHostToDevice, stream));
+        H2_TRY(hipMemcpyAsync(val, (const uint8_t*)values[c].ptr + 32 * offset, 32 * length, hipMemcpyHostToDevice,
+            stream));
+        FFITraitObject p { (uint64_t*)perm }, v { (uint64_t*)val };
+        // one column at a time through the device routine (offset already applied)
+        RustError e = permutation_product_columns(den, num, &p, &v, (const Fr*)beta_device, (const Fr*)gamma_device,
+            (const Fr*)delta_device, (const Fr*)omega_device, (Fr*)deltaomega_device, length, 0, 1, c == 0, stream);
+        if (e.code)
+            return e;
+    }
+    RustError e = permutation_product_finish(den, num, length, stream);
+    if (e.code)
+        return e;
+    H2_TRY(hipMemcpyAsync(modified_values->ptr, den, 32 * length, hipMemcpyDeviceToHost, stream));
+    H2_TRY(hipStreamSynchronize(stream));
+    return ok();
+}
+
+uint64_t _halo2_permutation_product_device_inputs_workspace_size(uint64_t poly_length)
+{
+    return align_up(32 * poly_length, 256) + 256;
+}
+
+RustError _halo2_permutation_product_device_inputs(const FFITraitObject* modified_values_device,
+    const FFITraitObject* permutations_device, const FFITraitObject* values_device, const void* beta_device,
+    const void* gamma_device, const void* delta_device, const void* omega_device, void* deltaomega_device,
+    size_t length, size_t offset, size_t batch, void* scratch, uint64_t scratch_bytes, void* stream)
+{
+    if (length == 0)
+        return ok();
+    Workspace ws(scratch, scratch_bytes);
+    Fr* num = (Fr*)ws.take(32 * length);
+    if (ws.overflow)
+        return fail((int)hipErrorInvalidValue,
+            "_halo2_permutation_product_device_inputs: workspace of %lu bytes is too small",
+            (unsigned long)scratch_bytes);
+    return permutation_product_device((Fr*)modified_values_device->ptr, num, permutations_device, values_device,
+        (const Fr*)beta_device, (const Fr*)gamma_device, (const Fr*)delta_device, (const Fr*)omega_device,
+        (Fr*)deltaomega_device, length, offset, batch, (hipStream_t)stream);
+}
+
+// ------------------------------------------------------ lookup product
+
+size_t _halo2_commit_product_max_len(size_t length, uint64_t free_bytes)
+{
+    for (uint64_t l = length; l > 0; l >>= 1)
+        if (5 * 32 * l < free_bytes)
+            return l;
+    return 0;
+}
+
+uint64_t _halo2_commit_product_workspace_size(uint64_t poly_length) { return 5 * align_up(32 * poly_length, 256) + 256; }
+
+static RustError commit_product_device(Fr* lp, const Fr* pi, const Fr* pt, const Fr* ci, const Fr* ct, const Fr* beta,
+    const Fr* gamma, uint64_t n, hipStream_t stream)
+{
+    lookup_denominator_kernel<<<ew_grid(n), kEwBlock, 0, stream>>>(lp, pi, pt, beta, gamma, n);
+    H2_LAUNCH_CHECK();
+    RustError e = batch_invert<Fr>(lp, n, stream);
+    if (e.code)
+        return e;
+    lookup_numerator_kernel<<<ew_grid(n), kEwBlock, 0, stream>>>(lp, ci, ct, beta, gamma, n);
+    H2_LAUNCH_CHECK();
+    return ok();
+}
+
+RustError _halo2_commit_product(const FFITraitObject* lookup_product, const FFITraitObject* permuted_input,
+    const FFITraitObject* permuted_table, const FFITraitObject* compressed_input,
+    const FFITraitObject* compressed_table, const void* beta_device, const void* gamma_device, size_t length,
+    size_t offset, void* scratch, uint64_t scratch_bytes, void* stream_)
+{
+    hipStream_t stream = (hipStream_t)stream_;
+    if (length == 0)
+        return ok();
+    Workspace ws(scratch, scratch_bytes);
+    Fr* d[5];
+    for (int i = 0; i < 5; ++i)
+        d[i] = (Fr*)ws.take(32 * length);
+    if (ws.overflow)
+        return fail((int)hipErrorInvalidValue, "_halo2_commit_product: workspace of %lu bytes is too small",
+            (unsigned long)scratch_bytes);
+    const FFITraitObject* src[4] = { permuted_input, permuted_table, compressed_input, compressed_table };
+    for (int i = 0; i < 4; ++i)
+        H2_TRY(hipMemcpyAsync(d[i + 1], (const uint8_t*)src[i]->ptr + 32 * offset, 32 * length, hipMemcpyHostToDevice,
+            stream));
+    RustError e = commit_product_device(d[0], d[1], d[2], d[3], d[4], (const Fr*)beta_device, (const Fr*)gamma_device,
+        length, stream);
+    if (e.code)
+        return e;
+    H2_TRY(hipMemcpyAsync((uint8_t*)lookup_product->ptr + 32 * offset, d[0], 32 * length, hipMemcpyDeviceToHost,
+        stream));
+    H2_TRY(hipStreamSynchronize(stream));
+    return ok();
+}
+
+RustError _halo2_commit_product_device_inputs(const FFITraitObject* d_lookup_product,
+    const FFITraitObject* d_permuted_input, const FFITraitObject* d_permuted_table,
+    const FFITraitObject* d_compressed_input, const FFITraitObject* d_compressed_table, const void* beta_device,
+    const void* gamma_device, size_t length, size_t offset, void* stream)
+{
+    if (length == 0)
+        return ok();
+    return commit_product_device((Fr*)d_lookup_product->ptr + offset, (const Fr*)d_permuted_input->ptr + offset,
+        (const Fr*)d_permuted_table->ptr + offset, (const Fr*)d_compressed_input->ptr + offset,
+        (const Fr*)d_compressed_table->ptr + offset, (const Fr*)beta_device, (const Fr*)gamma_device, length,
+        (hipStream_t)stream);
+}
+
+} // extern "C"

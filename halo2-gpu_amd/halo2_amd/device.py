@@ -1,0 +1,44 @@
+"""Device-memory plumbing over torch (allocation, copies, the current stream).
+
+Field elements travel as (n, 4) uint64 numpy arrays on the host and as int64
+torch tensors (same bits) on the device; points as (n, 8).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+
+def device() -> torch.device:
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def empty_u64(n_words: int) -> torch.Tensor:
+    return torch.empty(max(1, int(n_words)), dtype=torch.int64, device=device())
+
+
+def empty_bytes(n_bytes: int) -> torch.Tensor:
+    """uint8 scratch rounded up to 256 bytes (hipMalloc alignment is >= 256)."""
+    n = max(256, (int(n_bytes) + 255) // 256 * 256)
+    return torch.empty(n, dtype=torch.uint8, device=device())
+
+
+def to_device(a: np.ndarray) -> torch.Tensor:
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    return torch.from_numpy(a.view(np.int64).reshape(-1).copy()).to(device())
+
+
+def to_host(t: torch.Tensor, cols: int = 4) -> np.ndarray:
+    return t.detach().cpu().numpy().view(np.uint64).reshape(-1, cols).copy()
+
+
+def ptr(t: torch.Tensor) -> int:
+    return int(t.data_ptr())
+
+
+def stream() -> int:
+    return int(torch.cuda.current_stream().cuda_stream)
+
+
+def sync() -> None:
+    torch.cuda.synchronize()

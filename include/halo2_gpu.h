@@ -1,0 +1,230 @@
+/*
+ * halo2_gpu.h - C ABI of the MI355X (gfx950) hot path of the halo2-axiom-gpu
+ * KZG prover.
+ *
+ * Every `_halo2_*` symbol below replaces the entry point of the same name in
+ * the reference's `extern "C"` block (halo2_proofs/src/cuda/culib.rs, cited
+ * per symbol), with the same argument order and meaning. A Rust caller keeps
+ * its `#[link(name = "halo2_gpu")]` declarations; only `cudaStream_t` becomes a
+ * HIP stream (`void*` here, `*mut c_void` on the Rust side).
+ *
+ * Conventions (as in the reference, culib.rs:1-10 and
+ * cuda/include/common/halo2_ffi.h:8-60):
+ *  - Field elements (Fr, Fq) are 32 bytes: 4 little-endian u64 limbs holding
+ *    a * 2^256 mod m (Montgomery form), fully reduced.
+ *  - G1 affine points are 64 bytes (x, y); the identity is (0, 0). MSM results
+ *    are 96 bytes (x, y, 1) or the identity (0, 1, 0), in Montgomery form.
+ *  - `void*` data arguments are caller-owned device pointers unless the name or
+ *    the comment says host. `FFITraitObject*` points at one `{uint64_t* ptr}`
+ *    word, or at an array of them for "many" entry points.
+ *  - Launchers enqueue on `stream` and never call hipSetDevice. Entry points
+ *    whose result lands in host memory synchronise `stream` before returning;
+ *    device-output entry points do not.
+ *  - Scratch memory is caller-owned: `_workspace_size` returns the bytes the
+ *    matching launcher carves from `scratch`; a short workspace is reported as
+ *    an error before anything is enqueued. The library never allocates device
+ *    memory on these paths.
+ *  - Errors come back as a 16-byte `RustError` by value: code 0 = success;
+ *    otherwise `message` is a malloc'd string the caller frees with free().
+ */
+#ifndef HALO2_GPU_H
+#define HALO2_GPU_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* cuda/include/common/halo2_ffi.h:30-48 ; src/cuda/error.rs:17-35 */
+typedef struct RustError {
+    int32_t code;
+    char* message;
+} RustError;
+
+/* cuda/include/common/halo2_ffi.h:51-60 ; src/cuda/utils.rs:98-152 */
+typedef struct FFITraitObject {
+    uint64_t* ptr;
+} FFITraitObject;
+
+/* NTT kinds, cuda/src/ntt.cu:21-28 and src/poly/domain.rs:104-116 */
+enum {
+    HALO2_NTT_FFT = 1,
+    HALO2_NTT_IFFT = 2,
+    HALO2_NTT_COSET_FFT = 3,
+    HALO2_NTT_ICOSET_FFT = 4,
+    HALO2_NTT_IFFT_COSET_FFT = 5, /* unsupported, as in the reference */
+    HALO2_NTT_COSET_FFT_PART = 6,
+};
+
+/* =========================================================== MSM (G1) */
+
+/* culib.rs:30 ; cuda/src/multiexp.cu:166-171 */
+size_t _halo2_msm_max_length(uint64_t free_bytes);
+/* culib.rs:32 ; multiexp.cu:174-178 */
+uint64_t _halo2_multiexp_workspace_size(uint64_t length);
+/* culib.rs:35-43 ; multiexp.cu:258-335 — host scalars, host bases, host output */
+RustError _halo2_multiexp(const FFITraitObject* scalar, const FFITraitObject* point,
+    const FFITraitObject* output, size_t length, void* scratch, uint64_t scratch_bytes, void* stream);
+/* culib.rs:45 ; multiexp.cu:341-345 */
+uint64_t _halo2_multiexp_device_bases_workspace_size(uint64_t length);
+/* culib.rs:48-56 ; multiexp.cu:355-424 — host scalars, device bases, host output */
+RustError _halo2_multiexp_device_bases(const FFITraitObject* h_scalar, const void* d_bases,
+    const FFITraitObject* h_output, size_t length, void* scratch, uint64_t scratch_bytes, void* stream);
+/* culib.rs:58 ; multiexp.cu:431-435 */
+uint64_t _halo2_multiexp_device_scalars_device_bases_workspace_size(uint64_t length);
+/* culib.rs:62-70 ; multiexp.cu:447-525 — device scalars, device bases, host output */
+RustError _halo2_multiexp_device_scalars_device_bases(const void* d_scalar, const void* d_bases,
+    const FFITraitObject* h_output, size_t length, void* scratch, uint64_t scratch_bytes, void* stream);
+
+/* ================================================================ NTT */
+
+/* culib.rs:454 ; cuda/src/ntt.cu:383-393 */
+uint64_t _halo2_fft_normal_workspace_size(uint32_t ntt_type, uint32_t log_n, uint32_t extend_log_n);
+/* culib.rs:456-466 ; ntt.cu:395-474 — host input/output/omega/divisor; FFT, iFFT, cosetFFT,
+ * icosetFFT and CosetFFT_Part at size 2^log_n */
+RustError _halo2_fft_normal(uint32_t ntt_type, uint32_t log_n, const void* input, void* output,
+    const void* omega, const void* divisor, void* scratch, uint64_t scratch_bytes, void* stream);
+/* culib.rs:468-473 ; ntt.cu:149-164 — always true (deterministic sizing) */
+bool _halo2_fft_normal_check_memory(uint32_t ntt_type, const void* input, uint32_t log_n,
+    uint32_t extend_log_n);
+/* culib.rs:475-479 ; ntt.cu:480-487 */
+uint64_t _halo2_fft_normal_to_device_workspace_size(uint32_t ntt_type, uint32_t log_n,
+    uint32_t extend_log_n);
+/* culib.rs:484-494 ; ntt.cu:509-572 — device input/output (may alias), device omega,
+ * host divisor; FFT, iFFT and CosetFFT_Part only */
+RustError _halo2_fft_normal_to_device(uint32_t ntt_type, uint32_t log_n, const void* d_input,
+    void* d_output, const void* d_omega, const void* h_divisor, void* scratch,
+    uint64_t scratch_bytes, void* stream);
+/* culib.rs:496-507 ; ntt.cu:576-639 — host input (2^log_n) -> host output (2^extend_log_n):
+ * zero-pad, multiply index i by zeta^(i mod 3), DFT with the extended omega */
+RustError _halo2_cosetfft(uint32_t ntt_type, uint32_t log_n, uint32_t extend_log_n, const void* input,
+    void* output, const void* omega, const void* divisor, void* scratch, uint64_t scratch_bytes,
+    void* stream);
+/* culib.rs:509 ; ntt.cu:749-760 */
+uint64_t _halo2_fft_many_workspace_size(uint32_t ntt_type, uint32_t log_n, uint32_t extend_log_n);
+/* culib.rs:511-515 ; ntt.cu:765-775 */
+uint64_t _halo2_fft_many_to_device_workspace_size(uint32_t ntt_type, uint32_t log_n,
+    uint32_t extend_log_n);
+/* culib.rs:522-534 ; ntt.cu:813-886 — host inputs/outputs, device omega, host divisor */
+RustError _halo2_fft_many(uint32_t ntt_type, uint32_t num_many, uint32_t log_n, uint32_t extend_log_n,
+    const FFITraitObject* input, const FFITraitObject* output, const void* d_omega,
+    const FFITraitObject* divisor, void* scratch, uint64_t scratch_bytes, void* stream);
+/* culib.rs:540-553 ; ntt.cu:898-962 — device outputs; inputs host or device per flag */
+RustError _halo2_fft_many_to_device(uint32_t ntt_type, uint32_t num_many, uint32_t log_n,
+    uint32_t extend_log_n, const FFITraitObject* input, const FFITraitObject* output,
+    const void* d_omega, const FFITraitObject* divisor, bool input_on_device, void* scratch,
+    uint64_t scratch_bytes, void* stream);
+/* culib.rs:555 ; ntt.cu:780-786 — always true */
+bool _halo2_fft_many_check_memory(uint32_t ntt_type, uint32_t log_n, uint32_t extend_log_n);
+/* culib.rs:557-562 ; ntt.cu:666-681 — smallest split (log2) whose working set fits, -1 if none */
+int32_t _halo2_get_fft_split_radix(uint32_t ntt_type, uint32_t log_n, uint32_t extend_log_n,
+    uint64_t free_bytes);
+
+/* ================================================ polynomial helpers */
+
+/* culib.rs:402-408 ; cuda/src/polynomial/polynomial.cu:247-267
+ * d_out[i * num_parts + p] = d_parts[p][i]; d_parts is a device array of device pointers */
+RustError _halo2_extended_from_lagrange_vec_device(void* d_out, const void* d_parts, uint32_t num_parts,
+    uint64_t n, void* stream);
+/* culib.rs:376-382 ; polynomial.cu:270-290 — d_poly[i] *= d_t_evals[i % t_len] */
+RustError _halo2_divide_by_vanishing_poly(void* d_poly, const void* d_t_evals, uint32_t t_len,
+    uint64_t n, void* stream);
+/* culib.rs:410-417 ; polynomial.cu:294-318
+ * d_a[i] *= d_coset_powers[(i % (len + 1)) - 1] when i % (len + 1) != 0 */
+RustError _halo2_distribute_powers_zeta(void* d_a, const void* d_coset_powers, uint32_t coset_powers_len,
+    uint64_t n, void* stream);
+
+/* ================================== products, scans and batch inversion */
+
+/* culib.rs:72-80 ; cuda/src/polynomial/operations.cu:16-42 — in place x -> 1/x, 0 -> 0;
+ * field_type 0 = Fr, 1 = Fq */
+RustError _halo2_batch_invert(void* d_data, uint32_t field_type, size_t length, void* stream);
+
+/* culib.rs:82 ; cuda/src/commitment/commit_product.cu:33-43 */
+size_t _halo2_commit_product_max_len(size_t length, uint64_t free_bytes);
+/* culib.rs:83 ; commit_product.cu:45-48 */
+uint64_t _halo2_commit_product_workspace_size(uint64_t poly_length);
+/* culib.rs:88-101 ; commit_product.cu:50-130 — host polynomials at [offset, offset + length),
+ * device beta/gamma: (A'+beta)(S'+gamma) inverted, times (A+beta)(S+gamma) */
+RustError _halo2_commit_product(const FFITraitObject* lookup_product, const FFITraitObject* permuted_input,
+    const FFITraitObject* permuted_table, const FFITraitObject* compressed_input,
+    const FFITraitObject* compressed_table, const void* beta_device, const void* gamma_device,
+    size_t length, size_t offset, void* scratch, uint64_t scratch_bytes, void* stream);
+/* culib.rs:115-126 ; commit_product.cu:142-186 — every .ptr is a device pointer */
+RustError _halo2_commit_product_device_inputs(const FFITraitObject* d_lookup_product,
+    const FFITraitObject* d_permuted_input, const FFITraitObject* d_permuted_table,
+    const FFITraitObject* d_compressed_input, const FFITraitObject* d_compressed_table,
+    const void* beta_device, const void* gamma_device, size_t length, size_t offset, void* stream);
+
+/* culib.rs:142 ; cuda/src/commitment/permutation_commit.cu:43-53 */
+size_t _halo2_permutation_product_max_len(size_t length, uint64_t free_bytes);
+/* culib.rs:143 ; permutation_commit.cu:55-58 */
+uint64_t _halo2_permutation_product_workspace_size(uint64_t poly_length);
+/* culib.rs:146-161 ; permutation_commit.cu:60-155 — host chunk buffers; modified_values is the
+ * running accumulator (length entries); permutations/values are arrays of `batch` host columns
+ * addressed at `offset`; deltaomega (device) is advanced by delta once per column */
+RustError _halo2_permutation_product(const FFITraitObject* modified_values, const FFITraitObject* permutations,
+    const FFITraitObject* values, const void* beta_device, const void* gamma_device, const void* delta_device,
+    const void* omega_device, void* deltaomega_device, size_t length, size_t offset, size_t batch,
+    void* scratch, uint64_t scratch_bytes, void* stream);
+/* culib.rs:163 ; permutation_commit.cu:175-178 */
+uint64_t _halo2_permutation_product_device_inputs_workspace_size(uint64_t poly_length);
+/* culib.rs:180-195 ; permutation_commit.cu:180-256 — device variant */
+RustError _halo2_permutation_product_device_inputs(const FFITraitObject* modified_values_device,
+    const FFITraitObject* permutations_device, const FFITraitObject* values_device, const void* beta_device,
+    const void* gamma_device, const void* delta_device, const void* omega_device, void* deltaomega_device,
+    size_t length, size_t offset, size_t batch, void* scratch, uint64_t scratch_bytes, void* stream);
+
+/* culib.rs:197 ; cuda/src/commitment/grand_product.cu:31-41 */
+size_t _halo2_grand_product_max_len(size_t length, uint64_t free_bytes);
+/* culib.rs:198 ; grand_product.cu:43-46 */
+uint64_t _halo2_grand_product_workspace_size(uint64_t poly_length);
+/* culib.rs:202-211 ; grand_product.cu:48-123 — output[i] = prefix * prod_{j<=i} input[j] over
+ * [offset, offset + length), host buffers, host prefix */
+RustError _halo2_grand_product(const FFITraitObject* output, const FFITraitObject* input,
+    const FFITraitObject* prefix, size_t length, size_t offset, void* scratch, uint64_t scratch_bytes,
+    void* stream);
+/* culib.rs:221-227 ; grand_product.cu:125-195 — in place on device, device prefix */
+RustError _halo2_grand_product_device_inputs(const FFITraitObject* d_inout, const FFITraitObject* d_prefix,
+    size_t length, size_t offset, void* stream);
+
+/* ===================================================== quotient pieces */
+
+/* culib.rs:564-579 ; cuda/src/quotient_lookups.cu:16-53 — all device pointers */
+RustError _halo2_quotient_lookups(void* values, const void* table_values, const void* product_coset,
+    const void* permuted_input_coset, const void* permuted_table_coset, const void* l0, const void* l_last,
+    const void* l_active_row, const void* beta, const void* gamma, const void* y, size_t poly_length,
+    void* stream);
+/* culib.rs:585-616 ; cuda/src/permutation_quotient.cu:23-93 — device vectors and pointer tables,
+ * host scalars */
+RustError _halo2_quotient_permutation(void* values, const void* l0, const void* l_last,
+    const void* l_active_row, const void* perm_prod_cosets, const void* perm_cosets,
+    const void* column_values, const void* beta, const void* gamma, const void* y, const void* delta,
+    const void* delta_start, const void* current_extended_omega, const void* omega, uint64_t n_sets,
+    uint64_t chunk_len, uint64_t n_perm_cols, int32_t last_rotation, int32_t rot_scale, int32_t isize_,
+    uint64_t poly_length, void* stream);
+
+/* ============================================= omega tables on device */
+
+/* culib.rs:427 ; cuda/src/polynomial/omega.cu */
+uint64_t _halo2_generate_omega_powers_workspace_size(uint32_t log_n);
+/* culib.rs:433-440 ; omega.cu — d_omega_powers[i] = omega^i for i < 2^log_n */
+RustError _halo2_generate_omega_powers(void* d_omega_powers, const void* d_omega, uint32_t log_n,
+    void* scratch, uint64_t scratch_bytes, void* stream);
+
+/* ======================== device data generation (no reference counterpart)
+ * Helpers used by the benchmark and tests to build large seeded inputs in HBM
+ * without a host round trip. */
+
+/* out[i] = Fr from ChaCha20 block (start + i) of the stream seeded like rand's
+ * `seed_from_u64(seed)` (64 uniform bytes reduced mod r, Montgomery form). */
+RustError h2amd_random_fr(void* d_out, uint64_t n, uint64_t seed, uint64_t start, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HALO2_GPU_H */
