@@ -1,0 +1,149 @@
+"""MSM wrappers over the C ABI, mirroring src/cuda/funcs/multiexp.rs and
+src/arithmetic.rs::best_multiexp of the reference, plus the KZG parameter
+object of src/poly/kzg/commitment.rs that keeps the SRS resident in HBM.
+
+Results are the reference's 96-byte projective-out layout ((n,) uint64[12]):
+(x, y, 1) or (0, 1, 0) in Montgomery form.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib, g1
+from . import device as dv
+from .fields import R_MOD, mont_limbs
+
+
+def _out_obj():
+    out = np.zeros(12, dtype=np.uint64)
+    return out, _lib.obj(out.ctypes.data)
+
+
+def multiexp_device_scalars_device_bases(d_scalars: torch.Tensor, d_bases: torch.Tensor, n: int,
+                                         stream: int | None = None) -> np.ndarray:
+    """`_halo2_multiexp_device_scalars_device_bases` (multiexp.rs:215-256): one chunk."""
+    lib = _lib.load()
+    ws = int(lib._halo2_multiexp_device_scalars_device_bases_workspace_size(n))
+    scratch = dv.empty_bytes(ws)
+    out, obj = _out_obj()
+    _lib.check(lib._halo2_multiexp_device_scalars_device_bases(dv.ptr(d_scalars), dv.ptr(d_bases),
+                                                               ctypes.byref(obj), n, dv.ptr(scratch), ws,
+                                                               stream if stream is not None else dv.stream()),
+               "_halo2_multiexp_device_scalars_device_bases")
+    return out
+
+
+def multiexp_device_bases(h_scalars: np.ndarray, d_bases: torch.Tensor) -> np.ndarray:
+    """`_halo2_multiexp_device_bases` (multiexp.rs:122-170): host scalars, resident bases."""
+    lib = _lib.load()
+    s = np.ascontiguousarray(h_scalars, dtype=np.uint64).reshape(-1, 4)
+    n = len(s)
+    ws = int(lib._halo2_multiexp_device_bases_workspace_size(n))
+    scratch = dv.empty_bytes(ws)
+    out, obj = _out_obj()
+    sobj = _lib.obj(s.ctypes.data)
+    _lib.check(lib._halo2_multiexp_device_bases(ctypes.byref(sobj), dv.ptr(d_bases), ctypes.byref(obj), n,
+                                                dv.ptr(scratch), ws, dv.stream()), "_halo2_multiexp_device_bases")
+    return out
+
+
+def multiexp(h_scalars: np.ndarray, h_points: np.ndarray) -> np.ndarray:
+    """`_halo2_multiexp` (multiexp.rs:35-108 single chunk): host scalars and bases."""
+    lib = _lib.load()
+    s = np.ascontiguousarray(h_scalars, dtype=np.uint64).reshape(-1, 4)
+    pts = np.ascontiguousarray(h_points, dtype=np.uint64).reshape(-1, 8)
+    assert len(s) == len(pts)
+    n = len(s)
+    ws = int(lib._halo2_multiexp_workspace_size(n))
+    scratch = dv.empty_bytes(ws)
+    out, obj = _out_obj()
+    so, po = _lib.obj(s.ctypes.data), _lib.obj(pts.ctypes.data)
+    _lib.check(lib._halo2_multiexp(ctypes.byref(so), ctypes.byref(po), ctypes.byref(obj), n, dv.ptr(scratch), ws,
+                                   dv.stream()), "_halo2_multiexp")
+    return out
+
+
+def multiexp_gpu_device_scalars_device_bases_chunked(d_scalars: torch.Tensor, d_bases: torch.Tensor, n: int,
+                                                     chunk_size: int) -> np.ndarray:
+    """multiexp.rs:258-308: split into chunks, fold the per-chunk points on the host."""
+    acc = None
+    for off in range(0, n, chunk_size):
+        m = min(chunk_size, n - off)
+        part = multiexp_device_scalars_device_bases(d_scalars[4 * off: 4 * (off + m)],
+                                                    d_bases[8 * off: 8 * (off + m)], m)
+        acc = g1.add(acc, g1.from_out12(part))
+    return g1.to_out12(acc)
+
+
+def best_multiexp(h_scalars: np.ndarray, h_points: np.ndarray) -> np.ndarray:
+    """src/arithmetic.rs:73-81. The reference switches to its CPU path below 2^14
+    points; here every size runs on the GPU (the result is the same point), with the
+    free-memory chunking of multiexp.rs:47-99."""
+    lib = _lib.load()
+    s = np.ascontiguousarray(h_scalars, dtype=np.uint64).reshape(-1, 4)
+    pts = np.ascontiguousarray(h_points, dtype=np.uint64).reshape(-1, 8)
+    n = len(s)
+    free, _ = torch.cuda.mem_get_info()
+    max_len = int(lib._halo2_msm_max_length(max(0, free - (256 << 20))))
+    if max_len == 0:
+        raise _lib.Halo2GpuError(2, "not enough device memory for any MSM chunk")
+    if n <= max_len:
+        return multiexp(s, pts)
+    acc = None
+    for off in range(0, n, max_len):
+        acc = g1.add(acc, g1.from_out12(multiexp(s[off: off + max_len], pts[off: off + max_len])))
+    return g1.to_out12(acc)
+
+
+def g1_mul_generator(d_scalars: torch.Tensor, n: int) -> torch.Tensor:
+    """d_out[i] = s_i * G (affine, 8 words per point) on the device."""
+    lib = _lib.load()
+    out = dv.empty_u64(8 * n)
+    ws = int(lib.h2amd_g1_mul_generator_workspace_size(n))
+    scratch = dv.empty_bytes(ws)
+    _lib.check(lib.h2amd_g1_mul_generator(dv.ptr(out), dv.ptr(d_scalars), n, dv.ptr(scratch), ws, dv.stream()),
+               "h2amd_g1_mul_generator")
+    return out
+
+
+def random_fr(n: int, seed: int, start: int = 0) -> torch.Tensor:
+    lib = _lib.load()
+    out = dv.empty_u64(4 * n)
+    _lib.check(lib.h2amd_random_fr(dv.ptr(out), n, seed, start, dv.stream()), "h2amd_random_fr")
+    return out
+
+
+def omega_powers(base: int, log_n: int) -> torch.Tensor:
+    """d[i] = base^i for i < 2^log_n (`_halo2_generate_omega_powers`)."""
+    lib = _lib.load()
+    out = dv.empty_u64(4 << log_n)
+    b = dv.to_device(mont_limbs(base % R_MOD).reshape(1, 4))
+    _lib.check(lib._halo2_generate_omega_powers(dv.ptr(out), dv.ptr(b), log_n, None, 0, dv.stream()),
+               "_halo2_generate_omega_powers")
+    return out
+
+
+class ParamsKZG:
+    """src/poly/kzg/commitment.rs:27-582: KZG parameters with the G1 SRS kept resident
+    in HBM (`g_device`). Built from a toxic-waste tau (`setup`, :146-230): g[i] = tau^i G.
+    The Lagrange basis and G2 elements are not part of this hot path."""
+
+    def __init__(self, k: int, tau: int):
+        self.k = k
+        self.n = 1 << k
+        self.tau = tau % R_MOD
+        self.g_device = g1_mul_generator(omega_powers(self.tau, k), self.n)
+
+    def commit_device(self, d_poly: torch.Tensor) -> np.ndarray:
+        """commitment.rs:546-577: MSM of device coefficients against the resident SRS."""
+        n = d_poly.numel() // 4
+        assert n <= self.n
+        return multiexp_device_scalars_device_bases(d_poly, self.g_device[: 8 * n], n)
+
+    def commit(self, h_poly: np.ndarray) -> np.ndarray:
+        """commitment.rs:456-470: host coefficients, resident SRS."""
+        h = np.ascontiguousarray(h_poly, dtype=np.uint64).reshape(-1, 4)
+        return multiexp_device_bases(h, self.g_device[: 8 * len(h)])

@@ -222,6 +222,11 @@ RustError _halo2_generate_omega_powers(void* d_omega_powers, const void* d_omega
 /* out[i] = Fr from ChaCha20 block (start + i) of the stream seeded like rand's
  * `seed_from_u64(seed)` (64 uniform bytes reduced mod r, Montgomery form). */
 RustError h2amd_random_fr(void* d_out, uint64_t n, uint64_t seed, uint64_t start, void* stream);
+/* d_out[i] = d_scalars[i] * G as 64-byte affine points (identity -> (0, 0)); scalars in
+ * Montgomery form, G = (1, 2). Builds KZG SRS points tau^i * G and s_i * G test bases. */
+uint64_t h2amd_g1_mul_generator_workspace_size(uint64_t n);
+RustError h2amd_g1_mul_generator(void* d_out, const void* d_scalars, uint64_t n, void* scratch,
+    uint64_t scratch_bytes, void* stream);
 
 #ifdef __cplusplus
 }

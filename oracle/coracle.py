@@ -113,6 +113,21 @@ def chacha_fr(seed: int, start: int, n: int) -> np.ndarray:
     return out
 
 
+def fr_inner_product(a, b) -> np.ndarray:
+    a, b = _c(a).reshape(-1, 4), _c(b).reshape(-1, 4)
+    out = np.zeros(4, dtype=np.uint64)
+    lib().or_fr_inner_product(_p(out), _p(a), _p(b), ctypes.c_size_t(len(a)))
+    return out
+
+
+def fr_powers(base, n: int) -> np.ndarray:
+    """out[i] = base^i (Montgomery)."""
+    from . import pyoracle
+
+    ones = np.tile(np.array(pyoracle.int_to_limbs(pyoracle.to_mont(1)), dtype=np.uint64), (n, 1))
+    return distribute_powers_inplace(ones, base)
+
+
 def g1_generator() -> np.ndarray:
     out = np.zeros(8, dtype=np.uint64)
     lib().or_g1_generator(_p(out))

@@ -1171,3 +1171,48 @@ void or_permutation_quotient(uint64_t* values, const uint64_t* const* zs, const 
         f_mul(&FR, beta_term, beta_term, omega);
     }
 }
+
+/* sum_i a[i] * b[i] in Fr (closed-form MSM checks: sum c_i (s_i G) = (sum c_i s_i) G) */
+typedef struct {
+    const uint64_t *a, *b;
+    uint64_t part[256][4];
+    size_t chunk;
+    size_t n;
+} ip_ctx;
+
+static void ip_range(void* vctx, size_t lo, size_t hi)
+{
+    ip_ctx* c = (ip_ctx*)vctx;
+    uint64_t acc[4] = { 0, 0, 0, 0 }, t[4];
+    for (size_t i = lo; i < hi; ++i) {
+        f_mul(&FR, t, c->a + 4 * i, c->b + 4 * i);
+        f_add(&FR, acc, acc, t);
+    }
+    f_copy(c->part[lo / c->chunk], acc);
+}
+
+void or_fr_inner_product(uint64_t* out, const uint64_t* a, const uint64_t* b, size_t n)
+{
+    ip_ctx* ctx = (ip_ctx*)calloc(1, sizeof(ip_ctx));
+    int nt = or_num_threads();
+    if (nt > 256)
+        nt = 256;
+    ctx->a = a;
+    ctx->b = b;
+    ctx->n = n;
+    ctx->chunk = (n + nt - 1) / nt;
+    if (ctx->chunk == 0)
+        ctx->chunk = 1;
+    if (n < 1024) {
+        ip_range(ctx, 0, n);
+        f_copy(out, ctx->part[0]);
+        free(ctx);
+        return;
+    }
+    parallel_for(n, nt, ip_range, ctx);
+    uint64_t acc[4] = { 0, 0, 0, 0 };
+    for (int t = 0; t < 256; ++t)
+        f_add(&FR, acc, acc, ctx->part[t]);
+    f_copy(out, acc);
+    free(ctx);
+}

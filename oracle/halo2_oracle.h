@@ -81,6 +81,8 @@ void or_permutation_quotient(uint64_t* values, const uint64_t* const* perm_prod_
                              size_t n_sets, size_t chunk_len, size_t n_perm_cols, int32_t last_rotation,
                              int32_t rot_scale, int32_t isize_, size_t n);
 
+void or_fr_inner_product(uint64_t* out, const uint64_t* a, const uint64_t* b, size_t n);
+
 int or_num_threads(void);
 
 #ifdef __cplusplus

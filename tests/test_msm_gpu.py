@@ -1,0 +1,175 @@
+"""GPU parity of the BN254 G1 MSM against the CPU oracle (bit-exact affine output),
+through the C ABI. Mirrors tests/multiexp_dense_bucket_edge_equivalence.rs of the
+reference (random, dense constant prefix, zero scalars + identity bases, all-zero,
+single non-zero, ten dense buckets per window, sub-tile dense chunk; chunked and
+unchunked) and src/cuda/tests.rs:390-585, plus closed forms at 2^20+:
+sum c_i (s_i G) = (sum c_i s_i) G and the KZG identity sum c_i tau^i G = f(tau) G."""
+import numpy as np
+import pytest
+
+from conftest import rand_fr
+
+pytestmark = pytest.mark.gpu
+
+
+def _dense_constant():
+    from halo2_amd import fields as F
+
+    v = F.from_limbs([0x9E3779B97F4A7C15, 0xBF58476D1CE4E5B9, 0x94D049BB133111EB, 0x022B5F6A3C1D0E8F])
+    return F.mont_limbs(v)
+
+
+def _ten_group_scalar(g):
+    from halo2_amd import fields as F
+
+    b = bytes((((g + j) % 250) + 1) for j in range(31)) + b"\x00"
+    return F.mont_limbs(int.from_bytes(b, "little"))
+
+
+def _check_all_paths(co, scalars, points, label, chunked=True):
+    """GPU device/device, device-bases, host/host and chunked vs the oracle MSM."""
+    from halo2_amd import device as dv, msm
+
+    n = len(scalars)
+    exp = co.msm(scalars, points)
+    ds, dp = dv.to_device(scalars), dv.to_device(points)
+    got = msm.multiexp_device_scalars_device_bases(ds, dp, n)
+    assert (got == exp).all(), f"{label}: device scalars/bases"
+    assert (dv.to_host(ds) == scalars).all(), "scalars must not be modified"
+    assert (msm.multiexp_device_bases(scalars, dp) == exp).all(), f"{label}: device bases"
+    assert (msm.multiexp(scalars, points) == exp).all(), f"{label}: host"
+    if chunked and n >= 8:
+        assert (msm.multiexp_gpu_device_scalars_device_bases_chunked(ds, dp, n, n // 4 + 1) == exp).all(), \
+            f"{label}: chunked"
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 17, 31, 32, 100, 1000, 1 << 14])
+def test_msm_random_small(lib, co, n):
+    print(f"msm random n={n}", flush=True)
+    s = rand_fr(co, n, 3000 + n)
+    c = rand_fr(co, n, 4000 + n)
+    pts = co.g1_mul_gen(s)
+    _check_all_paths(co, c, pts, f"random n={n}")
+
+
+def test_msm_edge_cases(lib, co):
+    from halo2_amd import fields as F
+
+    n = 1 << 14
+    k = _dense_constant()
+    s = rand_fr(co, n, 11)
+    c = rand_fr(co, n, 12)
+    pts = co.g1_mul_gen(s)
+    zero = np.zeros(4, dtype=np.uint64)
+    print("msm edge: zeros + identity bases", flush=True)
+    cz = c.copy()
+    for i in range(n):
+        if i % 3 == 0:
+            cz[i] = zero
+        elif i % 7 == 0:
+            cz[i] = k
+    pz = pts.copy()
+    pz[::5] = 0  # identity bases (0, 0)
+    _check_all_paths(co, cz, pz, "zeros + identity bases")
+    print("msm edge: all zero", flush=True)
+    allz = np.zeros_like(c)
+    _check_all_paths(co, allz, pts, "all-zero scalars")
+    from halo2_amd import msm, device as dv
+
+    out = msm.multiexp_device_scalars_device_bases(dv.to_device(allz), dv.to_device(pts), n)
+    assert (out[:4] == 0).all() and (out[4:8] == F.mont_limbs(1, F.P_MOD)).all() and (out[8:] == 0).all()
+    print("msm edge: single nonzero", flush=True)
+    single = np.zeros_like(c)
+    single[0] = k
+    _check_all_paths(co, single, pts, "single nonzero")
+    print("msm edge: dense prefix", flush=True)
+    dense = c.copy()
+    dense[: n // 4] = k
+    _check_all_paths(co, dense, pts, "dense 25% prefix")
+    print("msm edge: all equal (one bucket per window)", flush=True)
+    _check_all_paths(co, np.tile(k, (n, 1)), pts, "all equal")
+    print("msm edge: repeated points and P + (-P)", flush=True)
+    rep = np.tile(pts[5], (n, 1))
+    _check_all_paths(co, c, rep, "same base everywhere (doubling inside buckets)")
+    negs = pts.copy()
+    negs[1::2] = pts[0::2]
+    csame = np.tile(c[0], (n, 1))
+    _check_all_paths(co, csame, negs, "pairs of equal points, equal scalars")
+    # pairs (P, -P) with equal scalars: every pair cancels inside a bucket
+    inv = pts.copy()
+    for i in range(1, n, 2):
+        y = F.from_limbs(pts[i - 1, 4:8])
+        inv[i, :4] = pts[i - 1, :4]
+        inv[i, 4:8] = F.limbs((F.P_MOD - y) % F.P_MOD)
+    _check_all_paths(co, csame, inv, "pairs (P, -P) cancel")
+
+
+def test_msm_ten_dense_buckets_and_subtile(lib, co):
+    print("msm ten dense buckets per window", flush=True)
+    groups, per = 10, 6554
+    n = groups * per
+    c = np.concatenate([np.tile(_ten_group_scalar(g), (per, 1)) for g in range(groups)])
+    pts = co.g1_mul_gen(rand_fr(co, n, 21))
+    _check_all_paths(co, c, pts, "ten dense buckets / window", chunked=True)
+    print("msm sub-tile dense n=20", flush=True)
+    c20 = np.tile(_dense_constant(), (20, 1))
+    _check_all_paths(co, c20, co.g1_mul_gen(rand_fr(co, 20, 22)), "sub-tile dense", chunked=False)
+
+
+def test_msm_65536_vs_oracle(lib, co):
+    print("msm n=65536 vs oracle Pippenger", flush=True)
+    n = 1 << 16
+    s = rand_fr(co, n, 31)
+    c = rand_fr(co, n, 32)
+    pts = co.g1_mul_gen(s)
+    _check_all_paths(co, c, pts, "2^16", chunked=False)
+
+
+def test_g1_mul_generator_matches_oracle(lib, co):
+    from halo2_amd import device as dv, msm
+
+    n = 3000
+    s = rand_fr(co, n, 41)
+    s[7] = 0
+    got = dv.to_host(msm.g1_mul_generator(dv.to_device(s), n), cols=8)
+    assert (got == co.g1_mul_gen(s)).all()
+
+
+@pytest.mark.parametrize("k", [20, 22])
+def test_msm_closed_form_large(lib, co, k):
+    """sum c_i (s_i G) = (sum c_i s_i) G with device-generated bases and scalars."""
+    from halo2_amd import device as dv, msm
+
+    n = 1 << k
+    print(f"msm closed form n=2^{k}", flush=True)
+    ds = msm.random_fr(n, 51 + k)
+    dc = msm.random_fr(n, 61 + k)
+    pts = msm.g1_mul_generator(ds, n)
+    got = msm.multiexp_device_scalars_device_bases(dc, pts, n)
+    ip = co.fr_inner_product(dv.to_host(dc), dv.to_host(ds))
+    exp = co.g1_mul(co.g1_generator(), ip)
+    assert (got == exp).all()
+
+
+def test_kzg_commit_srs_identity(lib, co):
+    """ParamsKZG with a seeded tau: commit(f) = f(tau) G, resident SRS, device and host polys."""
+    from halo2_amd import device as dv, fields as F, msm
+
+    k = 18
+    n = 1 << k
+    tau = 0x1234567890ABCDEF1234567890ABCDEF % F.R_MOD
+    print(f"kzg commit k={k}", flush=True)
+    params = msm.ParamsKZG(k, tau)
+    poly = rand_fr(co, n, 71)
+    pows = co.fr_powers(F.mont_limbs(tau), n)
+    # the SRS itself: g[i] = tau^i G on a sample
+    g = dv.to_host(params.g_device, cols=8)
+    idx = [0, 1, 2, 12345, n - 1]
+    assert (g[idx] == co.g1_mul_gen(pows[idx])).all()
+    exp = co.g1_mul(co.g1_generator(), co.fr_inner_product(poly, pows))
+    assert (params.commit_device(dv.to_device(poly)) == exp).all()
+    assert (params.commit(poly) == exp).all()
+    # a shorter polynomial commits against the SRS prefix
+    m = 1000
+    exp_m = co.g1_mul(co.g1_generator(), co.fr_inner_product(poly[:m], pows[:m]))
+    assert (params.commit(poly[:m]) == exp_m).all()
