@@ -18,8 +18,11 @@
 //  6. Horner over windows (c doublings per window), conversion to affine.
 // The result is written as (x, y, 1) or the identity (0, 1, 0) in Montgomery
 // form, the layout of the reference's `cukernel_to_affine` output.
+#include <memory>
+
 #include "common.h"
 #include "curve.h"
+#include "profile.h"
 #include "scan_u32.h"
 
 using namespace h2;
@@ -372,6 +375,8 @@ RustError msm_device(const MsmPlan& p, const MsmBuffers& b, const Fr* d_scalars,
     hipStream_t stream)
 {
     const unsigned T = 256;
+    ProfScope prof_all("msm_total", stream);
+    std::unique_ptr<ProfScope> prof_sort(new ProfScope("msm_sort", stream));
     to_canonical_kernel<<<blocks_for(p.n, T), T, 0, stream>>>(d_scalars, b.canon, p.n);
     H2_LAUNCH_CHECK();
     H2_TRY(hipMemsetAsync(b.counts, 0, 4 * p.nb, stream));
@@ -383,6 +388,7 @@ RustError msm_device(const MsmPlan& p, const MsmBuffers& b, const Fr* d_scalars,
     H2_TRY(hipMemcpyAsync(b.cursor, b.off0, 4 * p.nb, hipMemcpyDeviceToDevice, stream));
     scatter_kernel<<<blocks_for(p.n, T), T, 0, stream>>>(b.canon, p.n, p.c, p.W, p.B, b.cursor, b.entries);
     H2_LAUNCH_CHECK();
+    prof_sort.reset();
 
     // level 0
     piece_count_kernel<<<blocks_for(p.nb, T), T, 0, stream>>>(b.off0, p.nb, p.S0, b.pc);
@@ -390,8 +396,11 @@ RustError msm_device(const MsmPlan& p, const MsmBuffers& b, const Fr* d_scalars,
     e = scan_u32_exclusive(b.pc, b.poff[0], p.nb, b.scan_scratch, stream);
     if (e.code)
         return e;
-    sum_level0_kernel<<<blocks_for(p.ub[0], T), T, 0, stream>>>(b.entries, d_points, b.off0, b.poff[0], p.nb, p.S0,
-        b.part[0]);
+    {
+        ProfScope prof("msm_bucket_level0", stream);
+        sum_level0_kernel<<<blocks_for(p.ub[0], T), T, 0, stream>>>(b.entries, d_points, b.off0, b.poff[0], p.nb,
+            p.S0, b.part[0]);
+    }
     H2_LAUNCH_CHECK();
     int cur = 0;
     for (uint32_t l = 1; l < p.levels; ++l) {
@@ -409,6 +418,7 @@ RustError msm_device(const MsmPlan& p, const MsmBuffers& b, const Fr* d_scalars,
     H2_LAUNCH_CHECK();
 
     // per-window weighted bucket sums
+    ProfScope prof_red("msm_reduce", stream);
     bucket_segment_kernel<<<blocks_for((uint64_t)p.W * p.M, T), T, 0, stream>>>(b.bsum, p.W, p.B, p.L, p.M, b.red[0]);
     H2_LAUNCH_CHECK();
     uint64_t Mi = p.M;

@@ -1,6 +1,7 @@
 // Fr NTT engine: table generation and the LDS-staged Stockham pass kernel.
 // See ntt_core.h for the decomposition.
 #include "ntt_core.h"
+#include "profile.h"
 
 namespace h2 {
 
@@ -347,7 +348,10 @@ RustError ntt_run(const NttTables& t, uint32_t log_n, const NttJob* jobs, int co
             if (T < 64)
                 T = 64;
             dim3 grid(nwg, nb);
-            ntt_pass_kernel<<<grid, T, 32u * RG, stream>>>(pb, pp);
+            {
+                ProfScope prof("ntt_pass", stream);
+                ntt_pass_kernel<<<grid, T, 32u * RG, stream>>>(pb, pp);
+            }
             H2_LAUNCH_CHECK();
         }
     }

@@ -95,7 +95,20 @@ SIGNATURES = {
     "h2amd_random_fr": (RustError, [P, U64, U64, U64, P]),
     "h2amd_g1_mul_generator_workspace_size": (U64, [U64]),
     "h2amd_g1_mul_generator": (RustError, [P, P, U64, P, U64, P]),
+    "h2amd_profile_enable": (None, [ctypes.c_int]),
+    "h2amd_profile_reset": (None, []),
+    "h2amd_profile_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(U64), ctypes.POINTER(ctypes.c_double)]),
+    "h2amd_mad_peak_probe": (RustError, [P, U32, U32, ctypes.POINTER(ctypes.c_double), P]),
 }
+
+
+def profile_read(name: str) -> tuple[int, float]:
+    """(launches, total milliseconds) recorded for a kernel name since the last reset."""
+    lib = load()
+    cnt = U64(0)
+    ms = ctypes.c_double(0.0)
+    lib.h2amd_profile_read(name.encode(), ctypes.byref(cnt), ctypes.byref(ms))
+    return int(cnt.value), float(ms.value)
 
 
 def load(path: str | None = None):

@@ -228,6 +228,16 @@ uint64_t h2amd_g1_mul_generator_workspace_size(uint64_t n);
 RustError h2amd_g1_mul_generator(void* d_out, const void* d_scalars, uint64_t n, void* scratch,
     uint64_t scratch_bytes, void* stream);
 
+/* ======================= kernel timing (no reference counterpart)
+ * When enabled, launchers bracket their hot kernels ("ntt_pass", "msm_bucket_level0",
+ * ...) with HIP events on the launch stream; reading syncs the recorded events. */
+void h2amd_profile_enable(int on);
+void h2amd_profile_reset(void);
+int h2amd_profile_read(const char* kernel, uint64_t* launches, double* total_ms);
+/* Measured v_mad_u64_u32 throughput (the integer multiply-add the field arithmetic
+ * runs on); scratch holds blocks * 256 * 8 bytes. */
+RustError h2amd_mad_peak_probe(void* scratch, uint32_t blocks, uint32_t iters, double* mads_per_s, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
