@@ -90,6 +90,16 @@ SIGNATURES = {
     "_halo2_quotient_lookups": (RustError, [P, P, P, P, P, P, P, P, P, P, P, SZ, P]),
     "_halo2_quotient_permutation": (RustError, [P, P, P, P, P, P, P, P, P, P, P, P, P, P,
                                                 U64, U64, U64, I32, I32, I32, U64, P]),
+    "_halo2_evaluate_h_max_rows": (U64, [P] + [U64] * 12),
+    "_halo2_quotient_workspace_size": (U64, [P] + [U64] * 11),
+    "_halo2_quotient": (RustError, [OBJP, U64, OBJP, U64, OBJP, U64, FFITraitObject, U64, FFITraitObject, U64,
+                                    FFITraitObject, P, U64, P, U64, FFITraitObject, U64, U64, U64, U64, P, U64, P]),
+    "_halo2_quotient_device_columns": (RustError, [P, U64, P, U64, P, U64, FFITraitObject, U64, FFITraitObject, U64,
+                                                   FFITraitObject, P, U64, P, U64, FFITraitObject, U64, U64, U64,
+                                                   U64, P, U64, P]),
+    "_halo2_quotient_device_columns_device_out": (RustError, [P, U64, P, U64, P, U64, FFITraitObject, U64,
+                                                              FFITraitObject, U64, FFITraitObject, P, U64, P, U64,
+                                                              P, U64, P, U64, U64, U64, P, U64, P]),
     "_halo2_generate_omega_powers_workspace_size": (U64, [U32]),
     "_halo2_generate_omega_powers": (RustError, [P, P, U32, P, U64, P]),
     "h2amd_random_fr": (RustError, [P, U64, U64, U64, P]),

@@ -207,6 +207,46 @@ RustError _halo2_quotient_permutation(void* values, const void* l0, const void* 
     uint64_t chunk_len, uint64_t n_perm_cols, int32_t last_rotation, int32_t rot_scale, int32_t isize_,
     uint64_t poly_length, void* stream);
 
+/* Custom-gate quotient: compiled expression graph (128-bit CalcRules + 64-bit value-part
+ * rules, src/plonk/evaluation.rs:107-350) interpreted per row over [row_start, row_end)
+ * of an extended part of num_quotient rows; rotations wrap modulo num_quotient.
+ * `rules`, `value_part_rules`, `challenges`, `constants` and `expr_constants`
+ * ([0, 1, -1, 2, y]) are host memory. */
+/* src/plonk/evaluation.rs:1435-1449 ; cuda/src/quotient.cu:309-331 */
+uint64_t _halo2_evaluate_h_max_rows(const uint64_t* rules, uint64_t num_rules, uint64_t num_vp_rules,
+    uint64_t num_quotient, uint64_t num_fixed, uint64_t num_instance, uint64_t num_advice,
+    uint64_t num_challenges, uint64_t num_constants, uint64_t rotation_scale, uint64_t row_start,
+    uint64_t row_end, uint64_t free_bytes);
+/* evaluation.rs:1451-1464 ; quotient.cu:350-372 */
+uint64_t _halo2_quotient_workspace_size(const uint64_t* rules, uint64_t num_rules, uint64_t num_vp_rules,
+    uint64_t num_quotient, uint64_t num_fixed, uint64_t num_instance, uint64_t num_advice,
+    uint64_t num_challenges, uint64_t num_constants, uint64_t rotation_scale, uint64_t row_start,
+    uint64_t row_end);
+/* evaluation.rs:1466-1490 ; quotient.cu:432-560 — host columns, host output */
+RustError _halo2_quotient(const FFITraitObject* fixed, uint64_t num_fixed, const FFITraitObject* instance,
+    uint64_t num_instance, const FFITraitObject* advices, uint64_t num_advice, FFITraitObject challenges,
+    uint64_t num_challenges, FFITraitObject constants, uint64_t num_constants, FFITraitObject expr_constants,
+    const uint64_t* rules, uint64_t num_rules, const uint64_t* value_part_rules, uint64_t num_vp_rules,
+    FFITraitObject quotient_poly, uint64_t num_quotient, uint64_t rotation_scale, uint64_t row_start,
+    uint64_t row_end, void* scratch, uint64_t scratch_bytes, void* stream);
+/* evaluation.rs:1537-1560 — host arrays of device column pointers, host output */
+RustError _halo2_quotient_device_columns(const void* const* fixed_d_ptrs, uint64_t num_fixed,
+    const void* const* instance_d_ptrs, uint64_t num_instance, const void* const* advice_d_ptrs,
+    uint64_t num_advice, FFITraitObject challenges, uint64_t num_challenges, FFITraitObject constants,
+    uint64_t num_constants, FFITraitObject expr_constants, const uint64_t* rules, uint64_t num_rules,
+    const uint64_t* value_part_rules, uint64_t num_vp_rules, FFITraitObject quotient_poly,
+    uint64_t num_quotient, uint64_t rotation_scale, uint64_t row_start, uint64_t row_end, void* scratch,
+    uint64_t scratch_bytes, void* stream);
+/* evaluation.rs:1499-1523 — device columns, device output (row r - row_start), optional device
+ * per-row Horner seed prev_values (null seeds zero) */
+RustError _halo2_quotient_device_columns_device_out(const void* const* fixed_d_ptrs, uint64_t num_fixed,
+    const void* const* instance_d_ptrs, uint64_t num_instance, const void* const* advice_d_ptrs,
+    uint64_t num_advice, FFITraitObject challenges, uint64_t num_challenges, FFITraitObject constants,
+    uint64_t num_constants, FFITraitObject expr_constants, const uint64_t* rules, uint64_t num_rules,
+    const uint64_t* value_part_rules, uint64_t num_vp_rules, void* quotient_poly_device_ptr,
+    uint64_t num_quotient, const void* prev_values_device_ptr, uint64_t rotation_scale, uint64_t row_start,
+    uint64_t row_end, void* scratch, uint64_t scratch_bytes, void* stream);
+
 /* ============================================= omega tables on device */
 
 /* culib.rs:427 ; cuda/src/polynomial/omega.cu */

@@ -271,3 +271,32 @@ def permutation_quotient(values, zs, perms, cols, l0, l_last, l_active, beta, ga
                                   ctypes.c_int32(last_rotation), ctypes.c_int32(rot_scale), ctypes.c_int32(isize),
                                   ctypes.c_size_t(len(out)))
     return out
+
+
+def quotient_rules(fixed, instance, advice, challenges, constants, y, rules, vp, nq: int, rot_scale: int,
+                   row_start: int, row_end: int, prev=None) -> np.ndarray:
+    """Oracle of the custom-gate rule interpreter over rows [row_start, row_end)."""
+    fixed = [_c(x).reshape(-1, 4) for x in fixed]
+    instance = [_c(x).reshape(-1, 4) for x in instance]
+    advice = [_c(x).reshape(-1, 4) for x in advice]
+    ch = _c(challenges).reshape(-1, 4) if len(challenges) else np.zeros((1, 4), dtype=np.uint64)
+    cs = _c(constants).reshape(-1, 4)
+    rules = _c(rules).reshape(-1)
+    if len(rules) == 0:
+        rules = np.zeros(2, dtype=np.uint64)
+        num_rules = 0
+    else:
+        num_rules = len(rules) // 2
+    num_vp = len(vp)
+    vp = _c(vp).reshape(-1) if num_vp else np.zeros(1, dtype=np.uint64)
+    out = np.zeros((row_end - row_start, 4), dtype=np.uint64)
+    pv = _c(prev).reshape(-1, 4) if prev is not None else None
+    empty = np.zeros((1, 4), dtype=np.uint64)
+    lib().or_quotient_rules(_p(out), _p(pv) if pv is not None else None,
+                            _ptr_array(fixed or [empty]), ctypes.c_size_t(len(fixed)),
+                            _ptr_array(instance or [empty]), ctypes.c_size_t(len(instance)),
+                            _ptr_array(advice or [empty]), ctypes.c_size_t(len(advice)),
+                            _p(ch), _p(cs), _p(_c(y).reshape(4)), _p(rules), ctypes.c_size_t(num_rules),
+                            _p(vp), ctypes.c_size_t(num_vp), ctypes.c_size_t(nq), ctypes.c_int64(rot_scale),
+                            ctypes.c_size_t(row_start), ctypes.c_size_t(row_end))
+    return out

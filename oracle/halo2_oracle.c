@@ -1216,3 +1216,79 @@ void or_fr_inner_product(uint64_t* out, const uint64_t* a, const uint64_t* b, si
     f_copy(out, acc);
     free(ctx);
 }
+
+/* Custom-gate rule interpreter, restating the row semantics of
+ * cuda/include/kernel/quotient.h:57-303 (equivalently Calculation::evaluate,
+ * src/plonk/evaluation.rs:215-266, folded with Horner in y as
+ * src/cpu/evaluator.rs:150-411 does): intermediate i = (C1 a + C2 b)(d ? b : 1),
+ * C = [0, 1, -1, 2]; out[r] = Horner_y(prev[r] or 0, value parts). */
+void or_quotient_rules(uint64_t* out, const uint64_t* prev, const uint64_t* const* fixed, size_t nf,
+                       const uint64_t* const* instance, size_t ni, const uint64_t* const* advice, size_t na,
+                       const uint64_t* challenges, const uint64_t* constants, const uint64_t* y,
+                       const uint64_t* rules, size_t num_rules, const uint64_t* vp, size_t num_vp, size_t nq,
+                       int64_t rot_scale, size_t row_start, size_t row_end)
+{
+    (void)nf;
+    (void)ni;
+    (void)na;
+    uint64_t* interm = (uint64_t*)calloc(num_rules ? num_rules : 1, 32);
+    for (size_t r = row_start; r < row_end; ++r) {
+        for (size_t i = 0; i < num_rules; ++i) {
+            uint64_t w[2] = { rules[2 * i], rules[2 * i + 1] };
+            uint64_t v[2][4];
+            for (int o = 0; o < 2; ++o) {
+                uint32_t src = (uint32_t)(w[o] & 0xf), idx = (uint32_t)((w[o] >> 4) & 0xfffff);
+                int64_t rot = (int64_t)((w[o] >> 24) & 0x7fff);
+                if ((w[o] >> 39) & 1)
+                    rot = -rot;
+                int64_t rr = ((int64_t)r + rot * rot_scale) % (int64_t)nq;
+                if (rr < 0)
+                    rr += (int64_t)nq;
+                const uint64_t* s;
+                static const uint64_t zero4[4] = { 0, 0, 0, 0 };
+                switch (src) {
+                case 0: s = fixed[idx] + 4 * rr; break;
+                case 1: s = instance[idx] + 4 * rr; break;
+                case 2: s = advice[idx] + 4 * rr; break;
+                case 3: s = interm + 4 * idx; break;
+                case 4: s = constants + 4 * idx; break;
+                case 5: s = challenges + 4 * idx; break;
+                default: s = zero4; break;
+                }
+                f_copy(v[o], s);
+            }
+            uint32_t c1 = (uint32_t)((w[0] >> 40) & 0xf), d = (uint32_t)((w[0] >> 44) & 0xf);
+            uint32_t c2 = (uint32_t)((w[1] >> 40) & 0xf);
+            uint64_t res[4] = { 0, 0, 0, 0 };
+            if (c1 == 1) {
+                if (c2 == 0) {
+                    if (d == 1)
+                        f_mul(&FR, res, v[0], v[1]);
+                    else
+                        f_copy(res, v[0]);
+                } else if (c2 == 1) {
+                    f_add(&FR, res, v[0], v[1]);
+                } else if (c2 == 2) {
+                    f_sub(&FR, res, v[0], v[1]);
+                }
+            } else if (c1 == 2) {
+                uint64_t z[4] = { 0, 0, 0, 0 };
+                f_sub(&FR, res, z, v[0]);
+            } else if (c1 == 3) {
+                f_add(&FR, res, v[0], v[0]);
+            }
+            f_copy(interm + 4 * i, res);
+        }
+        uint64_t val[4] = { 0, 0, 0, 0 };
+        if (prev)
+            f_copy(val, prev + 4 * (r - row_start));
+        for (size_t i = 0; i < num_vp; ++i) {
+            uint32_t src = (uint32_t)(vp[i] & 0xf), idx = (uint32_t)((vp[i] >> 4) & 0xfffff);
+            const uint64_t* part = src == 4 ? constants + 4 * idx : interm + 4 * idx;
+            f_mul(&FR, val, val, y);
+            f_add(&FR, val, val, part);
+        }
+        f_copy(out + 4 * (r - row_start), val);
+    }
+    free(interm);
+}

@@ -81,6 +81,12 @@ void or_permutation_quotient(uint64_t* values, const uint64_t* const* perm_prod_
                              size_t n_sets, size_t chunk_len, size_t n_perm_cols, int32_t last_rotation,
                              int32_t rot_scale, int32_t isize_, size_t n);
 
+void or_quotient_rules(uint64_t* out, const uint64_t* prev, const uint64_t* const* fixed, size_t nf,
+                       const uint64_t* const* instance, size_t ni, const uint64_t* const* advice, size_t na,
+                       const uint64_t* challenges, const uint64_t* constants, const uint64_t* y,
+                       const uint64_t* rules, size_t num_rules, const uint64_t* vp, size_t num_vp, size_t nq,
+                       int64_t rot_scale, size_t row_start, size_t row_end);
+
 void or_fr_inner_product(uint64_t* out, const uint64_t* a, const uint64_t* b, size_t n);
 
 int or_num_threads(void);
