@@ -1,30 +1,80 @@
 // Fr NTT engine: table generation and the LDS-staged Stockham pass kernel.
 // See ntt_core.h for the decomposition.
+//
+// Arithmetic inside the pass is the 9 x 29-bit lazy-reduction form of
+// field29.h: elements stay in the boundary's 2^256 Montgomery scale and every
+// constant the pass multiplies by (twiddles, coset powers, zeta, the iFFT
+// scale) is held in the 2^261 scale.
+#include "field29.h"
 #include "ntt_core.h"
 #include "profile.h"
 
 namespace h2 {
 
-// zeta (a primitive cube root of unity, halo2's g_coset) and zeta^2 in
-// Montgomery form; the same values the reference hard-codes at
-// cuda/include/kernel/zeta.h:19-30.
-__device__ __forceinline__ Fr zeta_pow(uint32_t m)
+// zeta (a primitive cube root of unity, halo2's g_coset) and zeta^2 times
+// 2^261 mod r, in 29-bit limbs. The reference hard-codes zeta and zeta^2 in
+// 2^256 Montgomery form at cuda/include/kernel/zeta.h:19-30; these are the
+// same field elements rescaled.
+__device__ __forceinline__ Fr29 zeta261(uint32_t m)
 {
-    Fr r;
-    if (m == 1) {
-        const uint32_t z[8] = { 0x55fcd653u, 0x0363f299u, 0x5fc1e200u, 0x73e7950bu, 0x576d9d24u, 0xc5fce83eu,
-            0xa1c3a4d4u, 0x059c805du };
+    constexpr uint32_t z1[9] = { 0x0f9aca5du, 0x06c39377u, 0x03fb9328u, 0x0dd12516u, 0x19b77045u, 0x1b53b497u,
+        0x086e5ab0u, 0x0b729bf7u, 0x00226320u };
+    constexpr uint32_t z2[9] = { 0x1065364du, 0x19a50e73u, 0x05348a9cu, 0x02af1c94u, 0x11265ae2u, 0x0a454b5au,
+        0x06633a88u, 0x1dbec294u, 0x000038f7u };
+    Fr29 r;
 #pragma unroll
-        for (int i = 0; i < 8; ++i)
-            r.v[i] = z[i];
-    } else {
-        const uint32_t z[8] = { 0x4a0329b3u, 0x93e7cedeu, 0x7a96c167u, 0x7d4fdca7u, 0xb19a750au, 0x8be4ba08u,
-            0xa5661c25u, 0x1cbd5653u };
-#pragma unroll
-        for (int i = 0; i < 8; ++i)
-            r.v[i] = z[i];
-    }
+    for (int i = 0; i < 9; ++i)
+        r.l[i] = m == 1 ? z1[i] : z2[i];
     return r;
+}
+
+// x * 2^256 -> x * 2^261 (mod r) on the host: one Montgomery product with
+// 2^261 mod r (CIOS over 32-bit words).
+static void host_to261(const Fr& x, uint32_t (&out)[9])
+{
+    const uint32_t* a = x.v;
+    const uint32_t* b = Fr29Params::TO261;
+    const uint32_t* m = FrParams::M;
+    uint32_t t[10] = { 0 };
+    for (int i = 0; i < 8; ++i) {
+        uint64_t c = 0;
+        for (int j = 0; j < 8; ++j) {
+            c += (uint64_t)t[j] + (uint64_t)a[j] * b[i];
+            t[j] = (uint32_t)c;
+            c >>= 32;
+        }
+        c += t[8];
+        t[8] = (uint32_t)c;
+        t[9] = (uint32_t)(c >> 32);
+        const uint32_t q = t[0] * FrParams::INV;
+        c = ((uint64_t)t[0] + (uint64_t)q * m[0]) >> 32;
+        for (int j = 1; j < 8; ++j) {
+            c += (uint64_t)t[j] + (uint64_t)q * m[j];
+            t[j - 1] = (uint32_t)c;
+            c >>= 32;
+        }
+        c += t[8];
+        t[7] = (uint32_t)c;
+        t[8] = t[9] + (uint32_t)(c >> 32);
+    }
+    // t < 2m: reduce once
+    uint32_t d[8];
+    int64_t br = 0;
+    for (int j = 0; j < 8; ++j) {
+        const int64_t v = (int64_t)t[j] - m[j] + br;
+        d[j] = (uint32_t)v;
+        br = v >> 32;
+    }
+    const bool ge = t[8] != 0 || br == 0;
+    uint32_t w[8];
+    for (int j = 0; j < 8; ++j)
+        w[j] = ge ? d[j] : t[j];
+    for (int i = 0; i < 8; ++i) {
+        const int wi = (29 * i) >> 5, s = (29 * i) & 31;
+        const uint64_t two = ((uint64_t)w[wi + 1] << 32) | w[wi];
+        out[i] = (uint32_t)(two >> s) & 0x1fffffffu;
+    }
+    out[8] = w[7] >> 8;
 }
 
 NttPlan ntt_plan(uint32_t log_n)
@@ -90,7 +140,8 @@ struct TableJobs {
     TableJob j[7];
 };
 
-// out[i] = (base^(2^shift))^i, base = omega (device) or c (by value)
+// out[i] = (base^(2^shift))^i * 2^261 / 2^256, base = omega (device) or c (by
+// value): the tables hold the 2^261-scale representatives the pass multiplies by.
 __global__ void ntt_tables_kernel(TableJobs tj, const Fr* d_omega, Fr c)
 {
     const TableJob job = tj.j[blockIdx.y];
@@ -100,7 +151,11 @@ __global__ void ntt_tables_kernel(TableJobs tj, const Fr* d_omega, Fr c)
     Fr b = job.use_c ? c : Fr::load(d_omega);
     for (uint32_t s = 0; s < job.shift; ++s)
         b = Fr::sqr(b);
-    Fr::pow_u64(b, i).store(job.out + i);
+    Fr to;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        to.v[k] = Fr29Params::TO261[k];
+    (Fr::pow_u64(b, i) * to).store(job.out + i);
 }
 
 RustError ntt_tables_build(Workspace& ws, NttTables& t, uint32_t log_n, const Fr* d_omega, bool powers_twist,
@@ -153,26 +208,35 @@ struct PassParams {
     int post;
     int last;
     uint32_t xcd_remap;
-    Fr scale;
+    uint32_t scale[9]; // iFFT scale in the 2^261 scale
 };
 
-__device__ __forceinline__ Fr lds_get(const uint4* lds, uint32_t i)
+// LDS holds the tile as 9 limb planes: limb q of slot i at lds[q * RG + i]
+__device__ __forceinline__ Fr29 lds_get(const uint32_t* lds, uint32_t RG, uint32_t i)
 {
-    uint4 a = lds[2 * i], b = lds[2 * i + 1];
-    Fr r;
-    r.v[0] = a.x; r.v[1] = a.y; r.v[2] = a.z; r.v[3] = a.w;
-    r.v[4] = b.x; r.v[5] = b.y; r.v[6] = b.z; r.v[7] = b.w;
+    Fr29 r;
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+        r.l[q] = lds[q * RG + i];
     return r;
 }
-__device__ __forceinline__ void lds_put(uint4* lds, uint32_t i, const Fr& x)
+__device__ __forceinline__ void lds_put(uint32_t* lds, uint32_t RG, uint32_t i, const Fr29& x)
 {
-    lds[2 * i] = make_uint4(x.v[0], x.v[1], x.v[2], x.v[3]);
-    lds[2 * i + 1] = make_uint4(x.v[4], x.v[5], x.v[6], x.v[7]);
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+        lds[q * RG + i] = x.l[q];
 }
 
+// Value bounds (m = r): the load writes normalised limbs of values < 1.25m
+// (inputs are < 2^256 and a twisted input is a product, < 1.25m). Stage 0
+// forms a + c and a + 2m - c; every later stage normalises a, multiplies c by
+// its twiddle (< 1.25m, normalised) and forms the same pair. So after stage s
+// every value is < (1.25 + 2s) m < 25.25m < 2^259 with limbs < 2^29 + 2^30,
+// within mul()'s operand limits; the write multiplies by the inter-pass
+// twiddle (or the scale), or reduces fully in the last pass.
 __global__ void __launch_bounds__(1024) ntt_pass_kernel(PassBatch pb, PassParams pp)
 {
-    extern __shared__ uint4 lds[];
+    extern __shared__ uint32_t lds[];
     const uint32_t log_r = pp.log_r, log_g = pp.log_g, log_l = pp.log_l;
     const uint32_t R = 1u << log_r;
     const uint32_t G = 1u << log_g;
@@ -191,26 +255,26 @@ __global__ void __launch_bounds__(1024) ntt_pass_kernel(PassBatch pb, PassParams
         const uint32_t g = f & (G - 1);
         const uint32_t n1 = f >> log_g;
         const uint64_t gi = (uint64_t)(j0 + g) + (uint64_t)n1 * stride;
-        Fr x;
+        Fr29 x;
         if (gi < pp.in_len) {
-            x = Fr::load(in + gi);
+            x = Fr29::load32(in + gi);
             if (pp.twist == TWIST_ZETA) {
-                uint32_t m = (uint32_t)(gi % 3u);
+                const uint32_t m = (uint32_t)(gi % 3u);
                 if (m)
-                    x = x * zeta_pow(m);
+                    x = Fr29::mul(x, zeta261(m));
             } else if (pp.twist == TWIST_POWERS) {
                 const uint32_t hm = (1u << pp.h) - 1;
                 const uint32_t lo = (uint32_t)gi & hm, hi = (uint32_t)(gi >> pp.h);
-                Fr w = Fr::load(pp.c_lo + lo);
+                Fr29 w = Fr29::load32(pp.c_lo + lo);
                 if (hi)
-                    w = w * Fr::load(pp.c_hi + hi);
-                x = x * w;
+                    w = Fr29::mul(w, Fr29::load32(pp.c_hi + hi));
+                x = Fr29::mul(x, w);
             }
         } else {
-            x = Fr::zero();
+            x = Fr29::zero();
         }
         const uint32_t slot = (g << log_r) + (__brev(n1) >> (32 - log_r));
-        lds_put(lds, slot, x);
+        lds_put(lds, RG, slot, x);
     }
 
     const uint32_t halfR = R >> 1;
@@ -224,16 +288,22 @@ __global__ void __launch_bounds__(1024) ntt_pass_kernel(PassBatch pb, PassParams
             const uint32_t blk = qq >> s;
             const uint32_t i0 = (g << log_r) + (blk << (s + 1)) + pos;
             const uint32_t i1 = i0 + half;
-            Fr a = lds_get(lds, i0);
-            Fr c = lds_get(lds, i1);
-            if (pos)
-                c = c * Fr::load(pp.twr + (pos << (log_r - 1 - s)));
-            lds_put(lds, i0, a + c);
-            lds_put(lds, i1, a - c);
+            Fr29 a = lds_get(lds, RG, i0);
+            Fr29 c = lds_get(lds, RG, i1);
+            if (s) {
+                a = Fr29::normalize(a);
+                c = Fr29::mul(c, Fr29::load32(pp.twr + (pos << (log_r - 1 - s))));
+            }
+            lds_put(lds, RG, i0, Fr29::add(a, c));
+            lds_put(lds, RG, i1, Fr29::sub2(a, c));
         }
     }
     __syncthreads();
 
+    Fr29 scale;
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+        scale.l[q] = pp.scale[q];
     const bool mapA = (G > 1) && (log_l > 0);
     const uint32_t lmask = (1u << log_l) - 1;
     for (uint32_t f = threadIdx.x; f < RG; f += T) {
@@ -245,30 +315,33 @@ __global__ void __launch_bounds__(1024) ntt_pass_kernel(PassBatch pb, PassParams
             k1 = f & (R - 1);
             g = f >> log_r;
         }
-        Fr x = lds_get(lds, (g << log_r) + k1);
+        Fr29 x = lds_get(lds, RG, (g << log_r) + k1);
         const uint32_t j = j0 + g;
         const uint32_t K = j & lmask;
         const uint32_t n2 = j >> log_l;
         const uint64_t o = ((uint64_t)n2 << (log_l + log_r)) + K + ((uint64_t)k1 << log_l);
         if (!pp.last) {
             const uint32_t e = (n2 * k1) << log_l;
-            if (e) {
-                const uint32_t hm = (1u << pp.h) - 1;
-                Fr w = Fr::load(pp.tw_lo + (e & hm));
-                const uint32_t hi = e >> pp.h;
-                if (hi)
-                    w = w * Fr::load(pp.tw_hi + hi);
-                x = x * w;
+            const uint32_t hm = (1u << pp.h) - 1;
+            Fr29 w = Fr29::load32(pp.tw_lo + (e & hm)); // entry 0 is one: the multiply also bounds x
+            const uint32_t hi = e >> pp.h;
+            if (hi)
+                w = Fr29::mul(w, Fr29::load32(pp.tw_hi + hi));
+            x = Fr29::mul(x, w);
+        } else {
+            if (pp.post != POST_NONE) {
+                x = Fr29::mul(x, scale);
+                if (pp.post == POST_SCALE_ZETA_INV) {
+                    const uint32_t m = (uint32_t)(o % 3u);
+                    if (m)
+                        x = Fr29::mul(x, zeta261(3u - m));
+                }
+            } else {
+                x = Fr29::normalize(x);
             }
-        } else if (pp.post != POST_NONE) {
-            x = x * pp.scale;
-            if (pp.post == POST_SCALE_ZETA_INV) {
-                uint32_t m = (uint32_t)(o % 3u);
-                if (m)
-                    x = x * zeta_pow(3u - m);
-            }
+            x = Fr29::reduce_full(x);
         }
-        x.store(out + o);
+        x.store32(out + o);
     }
 }
 
@@ -282,7 +355,7 @@ RustError ntt_run(const NttTables& t, uint32_t log_n, const NttJob* jobs, int co
     H2_CHECK_ARG(log_n >= 1 && log_n <= NTT_MAX_LOG, "NTT log_n=%u out of range [1, %u]", log_n, NTT_MAX_LOG);
     if (!g_attr_set) {
         H2_TRY(hipFuncSetAttribute((const void*)ntt_pass_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-            32 << NTT_MAX_LOG_R));
+            36 << NTT_MAX_LOG_R));
         g_attr_set = true;
     }
     const NttPlan p = ntt_plan(log_n);
@@ -296,6 +369,9 @@ RustError ntt_run(const NttTables& t, uint32_t log_n, const NttJob* jobs, int co
     int max_batch = NTT_MAX_BATCH;
     if (need_scratch && scratch_slots < (uint64_t)max_batch)
         max_batch = (int)scratch_slots;
+    uint32_t scale261[9] = { 0 };
+    if (post != POST_NONE)
+        host_to261(scale, scale261);
 
     for (int base = 0; base < count; base += max_batch) {
         const int nb = count - base < max_batch ? count - base : max_batch;
@@ -338,7 +414,8 @@ RustError ntt_run(const NttTables& t, uint32_t log_n, const NttJob* jobs, int co
             pp.twist = pass == 0 ? (int)twist : TWIST_NONE;
             pp.last = pass == p.P - 1;
             pp.post = pp.last ? (int)post : POST_NONE;
-            pp.scale = scale;
+            for (int q = 0; q < 9; ++q)
+                pp.scale[q] = scale261[q];
             const uint32_t RG = 1u << (pp.log_r + pp.log_g);
             const uint32_t nwg = (uint32_t)(N >> (pp.log_r + pp.log_g));
             pp.xcd_remap = (nwg % 8 == 0 && nwg >= 64) ? 1u : 0u;
@@ -350,7 +427,7 @@ RustError ntt_run(const NttTables& t, uint32_t log_n, const NttJob* jobs, int co
             dim3 grid(nwg, nb);
             {
                 ProfScope prof("ntt_pass", stream);
-                ntt_pass_kernel<<<grid, T, 32u * RG, stream>>>(pb, pp);
+                ntt_pass_kernel<<<grid, T, 36u * RG, stream>>>(pb, pp);
             }
             H2_LAUNCH_CHECK();
         }

@@ -296,3 +296,191 @@ def quotient_host(evaluator: GraphEvaluator, fixed, instance, advice, challenges
         _lib.check(lib._halo2_quotient(objs[0], len(fixed), objs[1], len(instance), objs[2], len(advice), *common),
                    "_halo2_quotient")
     return out
+
+
+# ------------------------------------------------------ column pool / compress
+
+
+class InsufficientGpuMemory(_lib.Halo2GpuError):
+    """HaloGpuError::InsufficientGpuMemory (src/cuda/error.rs): the all-at-once budget check failed."""
+
+    def __init__(self, context: str, magnitude: int, free_bytes: int):
+        super().__init__(-2, f"{context}: {magnitude} columns do not fit in {free_bytes} free bytes")
+        self.context = context
+        self.magnitude = magnitude
+        self.free_bytes = free_bytes
+
+
+RESERVED_BYTES = 256 << 20
+
+
+def query_device_free_bytes_for_chunking() -> int:
+    """Free device memory minus a 256 MiB reservation (src/cuda/utils.rs:173-198)."""
+    free, _ = torch.cuda.mem_get_info()
+    return max(0, int(free) - RESERVED_BYTES)
+
+
+class ColumnPool:
+    """Per-prove device column set (src/cuda/funcs/column_pool.rs:21-283): fixed / advice /
+    instance columns resident on the device with the pointer arrays the
+    `_halo2_quotient_device_columns*` entry points read. Columns are either uploaded
+    (`try_init`) or borrowed from caller-owned device tensors (`try_init_device`); the
+    memory check is all-at-once, before any upload, and a failed check leaves the pool
+    uninitialised so the caller can take its host arm."""
+
+    def __init__(self, n: int):
+        self.n = int(n)
+        self._owned: list[torch.Tensor] = []
+        self._ptrs = {"fixed": [], "advice": [], "instance": []}
+        self._arrays = {}
+        self.initialized = False
+
+    def is_initialized(self) -> bool:
+        return self.initialized
+
+    @staticmethod
+    def estimate_resident_bytes(n: int, n_fixed: int, n_advice: int, n_instance: int) -> int:
+        return 32 * int(n) * (n_fixed + n_advice + n_instance)
+
+    def is_gpu_memory_enough(self, n_fixed: int, n_advice: int, n_instance: int) -> bool:
+        total = self.estimate_resident_bytes(self.n, n_fixed, n_advice, n_instance)
+        return total < query_device_free_bytes_for_chunking()
+
+    def _check(self, context: str, nf: int, na: int, ni: int) -> None:
+        if not self.is_gpu_memory_enough(nf, na, ni):
+            raise InsufficientGpuMemory(context, nf + na + ni, query_device_free_bytes_for_chunking())
+
+    def _upload(self, kind: str, cols) -> None:
+        for c in cols:
+            c = np.ascontiguousarray(c, dtype=np.uint64)
+            assert c.shape == (self.n, 4), f"{kind} column shape {c.shape} != ({self.n}, 4)"
+            d = dv.to_device(c)
+            self._owned.append(d)
+            self._ptrs[kind].append(dv.ptr(d))
+
+    def _borrow(self, kind: str, cols) -> None:
+        for d in cols:
+            assert d.numel() * 8 >= 32 * self.n, f"{kind} device column shorter than {self.n} rows"
+            self._ptrs[kind].append(dv.ptr(d))
+
+    def _finish(self) -> None:
+        for kind, p in self._ptrs.items():
+            self._arrays[kind] = (ctypes.c_void_p * max(1, len(p)))(*p)
+        self.initialized = True
+
+    def try_init(self, fixed_values, advice_values, instance_values) -> None:
+        """Upload host columns ((n, 4) uint64 Montgomery words each). Idempotent."""
+        if self.initialized:
+            return
+        self._check("ColumnPool::try_init", len(fixed_values), len(advice_values), len(instance_values))
+        self._upload("fixed", fixed_values)
+        self._upload("advice", advice_values)
+        self._upload("instance", instance_values)
+        self._finish()
+
+    def try_init_device(self, pk_fixed_lagrange_mirror, fixed_values, advice_values_device,
+                        instance_values_device) -> None:
+        """Borrow device columns; fixed columns come from the proving key's device mirror
+        when it is present (and matches the host column count), else are uploaded."""
+        if self.initialized:
+            return
+        mirror = pk_fixed_lagrange_mirror
+        borrowed = mirror is not None and len(mirror) > 0 and (len(fixed_values) == 0
+                                                                or len(mirror) == len(fixed_values))
+        nf = 0 if borrowed else len(fixed_values)
+        self._check("ColumnPool::try_init_device", nf, 0, 0)
+        if borrowed:
+            self._borrow("fixed", mirror)
+        else:
+            self._upload("fixed", fixed_values)
+        self._borrow("advice", advice_values_device)
+        self._borrow("instance", instance_values_device)
+        self._finish()
+
+    def fixed_ptrs(self):
+        return self._arrays["fixed"]
+
+    def advice_ptrs(self):
+        return self._arrays["advice"]
+
+    def instance_ptrs(self):
+        return self._arrays["instance"]
+
+    def num_fixed(self) -> int:
+        return len(self._ptrs["fixed"])
+
+    def num_advice(self) -> int:
+        return len(self._ptrs["advice"])
+
+    def num_instance(self) -> int:
+        return len(self._ptrs["instance"])
+
+
+def for_compress(expressions) -> GraphEvaluator:
+    """GraphEvaluator::for_compress (evaluation.rs:640-652): every expression, then
+    Horner(Constant(0), parts, Y). The kernel starts the fold at zero and folds with
+    expr_constants[4], where the caller places theta."""
+    g = GraphEvaluator()
+    parts = [g.add_expression(e) for e in expressions]
+    g.add_calculation(("Horner", _vs("Constant", 0), tuple(parts), _vs("Y")))
+    return g
+
+
+def _pool_chunks(lib, rules, nvp: int, size: int, pool: ColumnPool, n_chal: int, n_const: int,
+                 rot_scale: int):
+    batch = int(lib._halo2_evaluate_h_max_rows(rules.ctypes.data, len(rules), nvp, size, pool.num_fixed(),
+                                               pool.num_instance(), pool.num_advice(), n_chal, n_const, rot_scale,
+                                               0, size, query_device_free_bytes_for_chunking()))
+    if batch <= 0:
+        raise _lib.Halo2GpuError(-2, "_halo2_evaluate_h_max_rows: no row fits in free device memory")
+    return [(s, min(size, s + batch)) for s in range(0, size, batch)]
+
+
+def _launch_pool(lib, graph: GraphEvaluator, expr_constants, size: int, rot_scale: int, pool: ColumnPool,
+                 challenges, out_host: np.ndarray | None, out_dev: torch.Tensor | None) -> None:
+    assert pool.is_initialized(), "ColumnPool is not initialised"
+    rules, vp = graph.encode_for_device()
+    rules = np.ascontiguousarray(rules)
+    nvp = len(vp)
+    assert nvp > 0
+    vp = np.ascontiguousarray(vp)
+    consts = _fr_host(graph.constants)
+    chals = _fr_host(challenges)
+    expr = _fr_host(expr_constants)
+    heads = (pool.fixed_ptrs(), pool.num_fixed(), pool.instance_ptrs(), pool.num_instance(), pool.advice_ptrs(),
+             pool.num_advice(), _lib.obj(chals.ctypes.data), len(challenges), _lib.obj(consts.ctypes.data),
+             len(graph.constants), _lib.obj(expr.ctypes.data), rules.ctypes.data, len(rules), vp.ctypes.data, nvp)
+    for r0, r1 in _pool_chunks(lib, rules, nvp, size, pool, len(challenges), len(graph.constants), rot_scale):
+        ws = int(lib._halo2_quotient_workspace_size(rules.ctypes.data, len(rules), nvp, size, pool.num_fixed(),
+                                                    pool.num_instance(), pool.num_advice(), len(challenges),
+                                                    len(graph.constants), rot_scale, r0, r1))
+        scratch = dv.empty_bytes(ws)
+        if out_host is not None:
+            chunk = out_host[r0:r1]
+            _lib.check(lib._halo2_quotient_device_columns(*heads, _lib.obj(chunk.ctypes.data), size, rot_scale, r0,
+                                                          r1, dv.ptr(scratch), ws, dv.stream()),
+                       "_halo2_quotient_device_columns")
+        else:
+            _lib.check(lib._halo2_quotient_device_columns_device_out(
+                *heads, dv.ptr(out_dev) + 32 * r0, size, None, rot_scale, r0, r1, dv.ptr(scratch), ws,
+                dv.stream()), "_halo2_quotient_device_columns_device_out")
+
+
+def compress_expressions_device(expressions, theta: int, size: int, rot_scale: int, pool: ColumnPool,
+                                challenges) -> np.ndarray:
+    """evaluation.rs:1582-1715: sum over expressions folded with theta
+    (acc = acc * theta + expr), evaluated on the device from the pool's columns, host output."""
+    lib = _lib.load()
+    graph = for_compress(expressions)
+    out = np.zeros((size, 4), dtype=np.uint64)
+    _launch_pool(lib, graph, [0, 1, R_MOD - 1, 2, theta], size, rot_scale, pool, challenges, out, None)
+    return out
+
+
+def compress_expressions_in_place_device(graph: GraphEvaluator, expr_constants, size: int, rot_scale: int,
+                                         pool: ColumnPool, challenges, d_out: torch.Tensor) -> None:
+    """evaluation.rs:1722-1830: a pre-built graph, caller-supplied expression constants
+    (slot 4 = the fold factor), device output in `d_out` (size rows)."""
+    lib = _lib.load()
+    assert d_out.numel() * 8 >= 32 * size
+    _launch_pool(lib, graph, expr_constants, size, rot_scale, pool, challenges, None, d_out)

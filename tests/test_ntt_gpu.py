@@ -115,6 +115,36 @@ def test_round_trip_and_linearity_2_24(lib):
         assert (F.value_of(hta[r]) + F.value_of(htb[r])) % F.R_MOD == F.value_of(hts[r])
 
 
+@pytest.mark.parametrize("k", [1, 6, 12, 13, 20, 24, 25])
+def test_fft_largest_stored_inputs(lib, co, po, k):
+    """Every input word = r - 1, the largest reduced representative (the worst case for
+    lazily reduced butterflies). Montgomery storage is linear, so the transform of a
+    constant has the closed form: FFT -> (N (r - 1), 0, ..., 0) and iFFT -> (r - 1, 0, ..., 0)
+    in stored words; CosetFFT_Part at small k is checked against the oracle."""
+    import torch
+
+    from halo2_amd import device as dv, fields as F, ntt
+
+    n = 1 << k
+    print(f"largest stored inputs k={k}", flush=True)
+    top = F.limbs(F.R_MOD - 1).astype(np.uint64)
+    d = torch.tensor(np.tile(top, n).view(np.int64), device="cuda")
+    out = dv.empty_u64(4 * n)
+    ntt.fft_normal_device(1, k, d, out, F.omega(k), 1)
+    h = dv.to_host(out)
+    assert F.from_limbs(h[0]) == (n * (F.R_MOD - 1)) % F.R_MOD
+    assert not h[1:].any()
+    ntt.fft_normal_device(2, k, d, out, F.inv(F.omega(k)), F.inv(n))
+    h = dv.to_host(out)
+    assert F.from_limbs(h[0]) == F.R_MOD - 1
+    assert not h[1:].any()
+    if k <= 13:
+        a = np.tile(top, n).reshape(n, 4)
+        c = F.ZETA
+        ntt.fft_normal_device(6, k, d, out, F.omega(k), c)
+        assert (dv.to_host(out) == _oracle_kind(co, po, 6, a, k, c=c)).all()
+
+
 def _limb_add_mod(x: np.ndarray, y: np.ndarray) -> np.ndarray:
     """Vectorised (x + y) mod r on Montgomery limbs (Montgomery form is additive)."""
     from halo2_amd import fields as F
