@@ -42,9 +42,88 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-msm", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-rows", action="store_true", help="skip the per-row measurements (products, quotient, ...)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one plain device copy of this size first (FETCH_SIZE/WRITE_SIZE calibration)")
     return ap.parse_args()
+
+
+def _timed(fn, reps, torch):
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def row_measurements(lib, dv, F, msm, torch, stream) -> dict:
+    """Device-resident timings of the other hot-path entry points at one large size each.
+    `gbs` uses the algorithmic bytes (32 B per element read or written, as the
+    reference's formats hold them); every row is VALU-bound, so `gbs` is far below HBM."""
+    from halo2_amd import _lib
+    from halo2_amd.domain import EvaluationDomain
+    from halo2_amd.quotient import Advice, Fixed, Product, Sum, custom_gates_evaluator, quotient_device
+
+    out = {}
+
+    def rec(name, n, dt, bytes_per_elem):
+        out[name] = {"n": n, "ms": round(dt * 1e3, 4), "elems_per_s": n / dt,
+                     "gbs": round(n * bytes_per_elem / dt / 1e9, 2)}
+
+    one = dv.to_device(F.mont_limbs(1).reshape(1, 4))
+    k = 24
+    n = 1 << k
+    d = msm.random_fr(n, seed=301)
+    rec("batch_invert_2^24", n, _timed(lambda: _lib.check(lib._halo2_batch_invert(dv.ptr(d), 0, n, stream)), 5,
+                                       torch), 64)
+    rec("grand_product_2^24", n, _timed(lambda: _lib.check(lib._halo2_grand_product_device_inputs(
+        ctypes.byref(_lib.obj(dv.ptr(d))), ctypes.byref(_lib.obj(dv.ptr(one))), n, 0, stream)), 5, torch), 64)
+    del d
+    k = 22
+    n = 1 << k
+    cols = [msm.random_fr(n, seed=310 + i) for i in range(5)]
+    beta, gamma = dv.to_device(F.mont_limbs(11).reshape(1, 4)), dv.to_device(F.mont_limbs(13).reshape(1, 4))
+    o = dv.empty_u64(4 * n)
+    objs = [_lib.obj(dv.ptr(x)) for x in [o] + cols[:4]]
+    rec("lookup_product_2^22", n, _timed(lambda: _lib.check(lib._halo2_commit_product_device_inputs(
+        *[ctypes.byref(x) for x in objs], dv.ptr(beta), dv.ptr(gamma), n, 0, stream)), 5, torch), 32 * 5)
+    perms = [msm.random_fr(n, seed=320 + i) for i in range(4)]
+    sc = [dv.to_device(F.mont_limbs(v).reshape(1, 4)) for v in (11, 13, 17, F.omega(k), 1)]
+    zmod = dv.empty_u64(4 * n)
+    pws = int(lib._halo2_permutation_product_device_inputs_workspace_size(n))
+    pscr = dv.empty_bytes(pws)
+
+    def perm():
+        zmod.copy_(cols[4])
+        _lib.check(lib._halo2_permutation_product_device_inputs(
+            ctypes.byref(_lib.obj(dv.ptr(zmod))), _lib.obj_array([dv.ptr(x) for x in perms]),
+            _lib.obj_array([dv.ptr(x) for x in cols[:4]]), *[dv.ptr(x) for x in sc], n, 0, 4, dv.ptr(pscr), pws,
+            stream))
+    rec("permutation_product_2^22_4cols", n, _timed(perm, 5, torch), 32 * 10)
+    # custom-gate quotient over one extended part: q_a a + q_b b + q_m a b + q_c c, rotations scaled by 4
+    a_, b_, c_ = Advice(0), Advice(1), Advice(2)
+    gate = Sum(Sum(Sum(Product(Fixed(0), a_), Product(Fixed(1), b_)), Product(Product(Fixed(2), a_), b_)),
+               Product(Fixed(3), c_))
+    ev = custom_gates_evaluator([gate])
+    fixed = [msm.random_fr(n, seed=330 + i) for i in range(4)]
+    rec("quotient_rules_standard_gate_2^22", n, _timed(
+        lambda: quotient_device(ev, fixed, [], cols[:3], [], 5, n, 4), 5, torch), 32 * (7 + 1))
+    del perms, fixed, zmod, pscr
+    # coeff_to_extended 2^20 -> 2^22 (pad, zeta twist, NTT of the extended size)
+    k = 20
+    dom = EvaluationDomain(5, k)  # quotient degree 4: extended domain 2^22
+    poly = msm.random_fr(1 << k, seed=340)
+    rec("coeff_to_extended_2^20_to_2^22", 1 << k,
+        _timed(lambda: dom.coeff_to_extended_many_device([poly]), 5, torch), 160)
+    del cols, poly
+    # MSM at 2^16 (the small-primitive size)
+    m = 1 << 16
+    pts = msm.g1_mul_generator(msm.random_fr(m, seed=350), m)
+    scal = msm.random_fr(m, seed=351)
+    rec("msm_2^16", m, _timed(lambda: msm.multiexp_device_scalars_device_bases(scal, pts, m), 5, torch), 96)
+    return out
 
 
 def main() -> int:
@@ -172,12 +251,16 @@ def main() -> int:
             extra["msm_2^%d_ms" % lk] = round(dt * 1e3, 3)
             extra["msm_2^%d_elems_per_s" % lk] = m / dt
             extra["msm_2^%d_valu_frac_of_mad_peak" % lk] = round(mads / dt / peak.value, 4)
-            for name in ("msm_sort", "msm_bucket_level0", "msm_reduce"):
+            for name in ("msm_sort", "msm_buckets", "msm_reduce"):
                 cnt, tot = _lib.profile_read(name)
                 if cnt:
                     extra["msm_2^%d_%s_ms" % (lk, name)] = round(tot / cnt, 3)
             del sc
         del srs
+
+    # ----------------------------------------- the other hot-path rows
+    if not args.no_rows:
+        extra["rows"] = row_measurements(lib, dv, F, msm, torch, stream)
 
     # ------------------------------------------------------ CPU baseline
     cpu = None
@@ -209,7 +292,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u32 (BN254 Fr, 8x32-bit Montgomery limbs)",
+        "dtype": "u32 (BN254 Fr: 32-byte Montgomery words in HBM, 9x29-bit limbs in registers)",
         "data": "synthetic (ChaCha20-seeded uniform Fr generated on device; SRS tau^i G from a seeded tau)",
         "config": {"workload": f"coset NTT (CosetFFT_Part, c = zeta) of 2^{k} Fr, device-resident",
                    "log_n": k, "parallelism": "single GPU"},

@@ -66,7 +66,7 @@ struct G1X29 {
     }
 
     // mdbl-2008-s: 2 (x, y), affine non-identity input
-    __device__ __noinline__ static G1X29 dbl_affine(const G1A29& a)
+    __device__ __forceinline__ static G1X29 dbl_affine(const G1A29& a)
     {
         const Fq29 U = Fq29::add(a.y, a.y); // < 4, limbs < 2^30
         const Fq29 V = Fq29::mul(U, U); // < 1.1
@@ -84,8 +84,10 @@ struct G1X29 {
         return r;
     }
 
-    // dbl-2008-s
-    __device__ __noinline__ static G1X29 dbl(const G1X29& p)
+    // dbl-2008-s (out of line: it is the rare branch of add)
+    __device__ __noinline__ static G1X29 dbl(const G1X29& p) { return dbl_inline(p); }
+
+    __device__ __forceinline__ static G1X29 dbl_inline(const G1X29& p)
     {
         if (p.is_inf())
             return p;

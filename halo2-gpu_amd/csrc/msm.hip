@@ -11,18 +11,22 @@
 //  3. sort of the (point index, sign) entries by bucket in two counting
 //     rounds whose ranks come from LDS atomics only:
 //       round 1 bins entries by (window, top bucket bits) per tile of points,
-//       round 2 splits each bin by the low FB bucket bits, chunk by chunk.
+//       round 2 splits each bin by the low FB bucket bits, chunk by chunk,
+//       staging the chunk in LDS so the final stores come out in runs.
 //     Per-tile / per-chunk histograms are laid out so that one linear
 //     exclusive scan yields every entry's destination, whatever the skew of
 //     the scalars (a bucket holding every point costs the same sort work).
-//  4. bucket sums with load balancing: every bucket is cut into pieces of at
-//     most S0 entries, one thread per piece accumulates XYZZ mixed additions.
-//     A bucket with one piece is final at once; 2..LIGHT pieces are summed by
-//     one thread; more pieces (dense buckets) get a work-group tree reduction.
-//  5. per window, sum_b b * B_b with running sums over segments of L1
-//     buckets, then levels of Lg segments: each level carries pairs (T, S)
-//     with  answer = sum T + sum_m m * S_m,  so no per-segment scalar
-//     multiplication is needed (the weights become log2 L doublings of S).
+//  4. bucket sums, balanced: thread t adds the S sorted entries
+//     [tS, tS + S) with XYZZ mixed additions, whatever the bucket boundaries
+//     (the next point is fetched while the current one is added). A bucket
+//     wholly inside one thread's range is final at once; the pieces of a
+//     bucket split between threads are merged afterwards (one thread for up
+//     to LIGHT pieces, a work-group tree for denser buckets).
+//  5. per window, sum_b b * B_b: running sums over segments of L1 buckets
+//     give pairs (T, S) with  answer = sum T + sum_m m * S_m;  work-group
+//     trees over 256 pairs keep that form (T = T_L + T_R + S_R,
+//     S = 2 (S_L + S_R), S carrying the block size as a weight) until one
+//     pair per window is left.
 //  6. Horner over windows (c doublings per window), conversion to affine.
 // The result is written as (x, y, 1) or the identity (0, 1, 0) in Montgomery
 // form, the layout of the reference's `cukernel_to_affine` output.
@@ -43,13 +47,19 @@ constexpr uint32_t R1_THREADS = 1024;
 constexpr uint32_t R1_PER = 4;
 constexpr uint32_t R1_TILE = R1_THREADS * R1_PER; // points per round-1 tile
 constexpr uint32_t R2_THREADS = 256;
-constexpr uint32_t R2_CHUNK = 16384; // entries per round-2 chunk
-constexpr uint32_t LIGHT = 32; // pieces summed by one thread
+constexpr uint32_t R2_CHUNK = 8192; // entries per round-2 chunk (staged in LDS)
+constexpr uint32_t LIGHT = 32; // pieces of a split bucket summed by one thread
 constexpr uint32_t HEAVY_THREADS = 256;
 constexpr uint32_t HEAVY_BLOCKS = 512;
-constexpr uint32_t RED_L1 = 8; // buckets per first-level reduction segment
-constexpr uint32_t RED_LG = 16; // segments per group at later levels
-constexpr uint32_t MAX_RED_LEVELS = 12;
+constexpr uint32_t RED_L1 = 16; // buckets per first-level reduction segment
+constexpr uint32_t TREE = 256; // threads (tree leaves) per work-group at later levels
+constexpr uint32_t TREE_K = 8; // pairs folded sequentially into each leaf
+constexpr uint32_t MAX_RED_LEVELS = 8;
+
+// 128-byte slot holding one XYZZ point (four 32-byte coordinates)
+struct PointSlot {
+    uint4 w[8];
+};
 
 struct MsmPlan {
     uint64_t n;
@@ -59,12 +69,12 @@ struct MsmPlan {
     uint32_t NC; // coarse bins = W << CB
     uint32_t ntiles; // round-1 tiles
     uint64_t max_chunks; // upper bound on round-2 chunks
-    uint32_t S0; // entries per level-0 piece
-    uint64_t ub0; // upper bound on pieces
+    uint32_t S; // sorted entries per accumulation thread
+    uint64_t nthr; // upper bound on accumulation threads
     uint32_t L1; // buckets per first reduction segment
-    uint32_t nlev; // reduction levels after the first
-    uint64_t Mlev[MAX_RED_LEVELS + 1]; // segments per window: Mlev[0] = B / L1, Mlev[i] after level i
-    uint32_t Llev[MAX_RED_LEVELS + 1]; // group size of level i (i >= 1)
+    uint32_t nlev; // tree levels after the first
+    uint64_t Mlev[MAX_RED_LEVELS + 1]; // pairs per window: Mlev[0] = B / L1, then / (leaves * K) per level
+    uint32_t Klev[MAX_RED_LEVELS + 1]; // pairs per leaf at level i (i >= 1)
 };
 
 uint32_t log2_floor(uint64_t n)
@@ -84,7 +94,9 @@ MsmPlan make_plan(uint64_t n)
         c = 4;
     if (c > 20)
         c = 20;
-    p.c = (uint32_t)c;
+    // same window count, narrowest width: the top window is as full as the others
+    const uint32_t w0 = (255 + (uint32_t)c - 1) / (uint32_t)c;
+    p.c = (255 + w0 - 1) / w0;
     p.W = (255 + p.c - 1) / p.c;
     p.B = 1u << (p.c - 1);
     p.nb = (uint64_t)p.W * p.B;
@@ -94,19 +106,27 @@ MsmPlan make_plan(uint64_t n)
     p.ntiles = (uint32_t)((n + R1_TILE - 1) / R1_TILE);
     const uint64_t work = n * p.W;
     p.max_chunks = work / R2_CHUNK + p.NC;
-    uint64_t s0 = (work + (1ull << 21) - 1) >> 21;
-    if (s0 < 4)
-        s0 = 4;
-    p.S0 = (uint32_t)s0;
-    p.ub0 = work / p.S0 + p.nb;
+    // ~2^21 accumulation threads, 16..128 entries each
+    uint64_t s = work >> 21;
+    if (s < 16)
+        s = 16;
+    if (s > 128)
+        s = 128;
+    p.S = (uint32_t)s;
+    p.nthr = (work + p.S - 1) / p.S;
     p.L1 = p.B < RED_L1 ? p.B : RED_L1;
     p.Mlev[0] = p.B / p.L1;
     uint32_t lev = 0;
     while (p.Mlev[lev] > 1) {
-        const uint32_t lg = p.Mlev[lev] < RED_LG ? (uint32_t)p.Mlev[lev] : RED_LG;
+        const uint64_t Mi = p.Mlev[lev];
         ++lev;
-        p.Llev[lev] = lg;
-        p.Mlev[lev] = p.Mlev[lev - 1] / lg;
+        if (Mi >= (uint64_t)TREE * TREE_K) {
+            p.Klev[lev] = TREE_K;
+            p.Mlev[lev] = Mi / ((uint64_t)TREE * TREE_K);
+        } else {
+            p.Klev[lev] = Mi > TREE ? (uint32_t)(Mi / TREE) : 1u;
+            p.Mlev[lev] = 1;
+        }
     }
     p.nlev = lev;
     return p;
@@ -117,21 +137,17 @@ struct MsmBuffers {
     uint32_t* canon; // n x 8 limbs
     uint32_t* h1; // NC * ntiles
     uint32_t* h1off; // NC * ntiles + 1
-    uint32_t* val1; // n * W (round-1 output: index | sign << 31)
-    uint16_t* key1; // n * W (round-1 output: low FB bucket bits)
+    uint64_t* kv1; // n * W round-1 output: (index | sign << 31) | low bucket bits << 32
     uint32_t* bin_start; // NC + 1
     uint32_t* chunk_base; // NC + 1
     uint32_t* h2; // max_chunks * 2^FB
     uint32_t* h2off; // max_chunks * 2^FB + 1
     uint32_t* entries; // n * W, sorted by bucket
     uint32_t* off0; // nb + 1
-    uint32_t* pc; // nb
-    uint32_t* poff; // nb + 1
-    uint32_t* piece_bucket; // ub0
-    uint32_t* heavy; // nb heavy bucket ids + 1 counter
-    G1Xyzz* part; // ub0
-    G1Xyzz* bsum; // nb
-    G1Xyzz* red[4]; // (T, S) ping-pong: W * Mlev[0] each
+    uint32_t* heavy; // nb queued split buckets + 1 counter
+    PointSlot* part; // 2 * nthr partial sums of split buckets
+    PointSlot* bsum; // nb
+    PointSlot* red[4]; // (T, S) ping-pong: W * Mlev[0] each
     void* scan_scratch;
     uint64_t* out; // 12 limbs
 };
@@ -144,8 +160,6 @@ void carve(WS& ws, const MsmPlan& p, MsmBuffers* b)
     uint64_t scan_n = nh1 + 1;
     if (nh2 + 1 > scan_n)
         scan_n = nh2 + 1;
-    if (p.nb + 1 > scan_n)
-        scan_n = p.nb + 1;
     const uint64_t nred = (uint64_t)p.W * p.Mlev[0];
     void* v[24];
     int k = 0;
@@ -153,19 +167,15 @@ void carve(WS& ws, const MsmPlan& p, MsmBuffers* b)
     v[k++] = ws.take(32 * p.n);
     v[k++] = ws.take(4 * nh1);
     v[k++] = ws.take(4 * (nh1 + 1));
-    v[k++] = ws.take(4 * p.n * p.W);
-    v[k++] = ws.take(2 * p.n * p.W);
+    v[k++] = ws.take(8 * p.n * p.W);
     v[k++] = ws.take(4 * (p.NC + 1));
     v[k++] = ws.take(4 * (p.NC + 1));
     v[k++] = ws.take(4 * nh2);
     v[k++] = ws.take(4 * (nh2 + 1));
     v[k++] = ws.take(4 * p.n * p.W);
     v[k++] = ws.take(4 * (p.nb + 1));
-    v[k++] = ws.take(4 * p.nb);
     v[k++] = ws.take(4 * (p.nb + 1));
-    v[k++] = ws.take(4 * p.ub0);
-    v[k++] = ws.take(4 * (p.nb + 1));
-    v[k++] = ws.take(128 * p.ub0);
+    v[k++] = ws.take(128 * 2 * p.nthr);
     v[k++] = ws.take(128 * p.nb);
     for (int i = 0; i < 4; ++i)
         v[k++] = ws.take(128 * nred);
@@ -177,22 +187,18 @@ void carve(WS& ws, const MsmPlan& p, MsmBuffers* b)
         b->canon = (uint32_t*)v[k++];
         b->h1 = (uint32_t*)v[k++];
         b->h1off = (uint32_t*)v[k++];
-        b->val1 = (uint32_t*)v[k++];
-        b->key1 = (uint16_t*)v[k++];
+        b->kv1 = (uint64_t*)v[k++];
         b->bin_start = (uint32_t*)v[k++];
         b->chunk_base = (uint32_t*)v[k++];
         b->h2 = (uint32_t*)v[k++];
         b->h2off = (uint32_t*)v[k++];
         b->entries = (uint32_t*)v[k++];
         b->off0 = (uint32_t*)v[k++];
-        b->pc = (uint32_t*)v[k++];
-        b->poff = (uint32_t*)v[k++];
-        b->piece_bucket = (uint32_t*)v[k++];
         b->heavy = (uint32_t*)v[k++];
-        b->part = (G1Xyzz*)v[k++];
-        b->bsum = (G1Xyzz*)v[k++];
+        b->part = (PointSlot*)v[k++];
+        b->bsum = (PointSlot*)v[k++];
         for (int i = 0; i < 4; ++i)
-            b->red[i] = (G1Xyzz*)v[k++];
+            b->red[i] = (PointSlot*)v[k++];
         b->scan_scratch = v[k++];
         b->out = (uint64_t*)v[k++];
     }
@@ -224,6 +230,21 @@ __global__ void to_canonical_kernel(const Fr* __restrict__ scal, uint32_t* __res
         return;
     Fr v = Fr::from_mont(Fr::load(scal + i));
     v.store(canon + 8 * i);
+}
+
+// bases -> 2^261 scale (x * 2^261 = Montgomery product of x * 2^256 and 2^261);
+// the identity (0, 0) stays (0, 0)
+__global__ void to_scale261_kernel(const uint8_t* __restrict__ in, uint8_t* __restrict__ out, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    Fq k;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        k.v[q] = Fq29Params::TO261[q];
+    (Fq::load(in + 64 * i) * k).store(out + 64 * i);
+    (Fq::load(in + 64 * i + 32) * k).store(out + 64 * i + 32);
 }
 
 // signed digit of window w; `carry` flows from window w-1 to w
@@ -286,9 +307,10 @@ __global__ void __launch_bounds__(R1_THREADS) r1_count_kernel(const uint32_t* __
         h1[(uint64_t)b * sp.ntiles + blockIdx.x] = hist[b];
 }
 
-// Round 1, pass 2: the same tile places its entries at the scanned offsets.
+// Round 1, pass 2: the same tile places its entries at the scanned offsets,
+// one 8-byte store per entry.
 __global__ void __launch_bounds__(R1_THREADS) r1_scatter_kernel(const uint32_t* __restrict__ canon, SortParams sp,
-    const uint32_t* __restrict__ h1off, uint32_t* __restrict__ val1, uint16_t* __restrict__ key1)
+    const uint32_t* __restrict__ h1off, uint64_t* __restrict__ kv1)
 {
     extern __shared__ uint32_t cursor[];
     for (uint32_t b = threadIdx.x; b < sp.NC; b += R1_THREADS)
@@ -310,8 +332,7 @@ __global__ void __launch_bounds__(R1_THREADS) r1_scatter_kernel(const uint32_t* 
             if (d != 0) {
                 const uint32_t bi = (uint32_t)(d < 0 ? -d : d) - 1u;
                 const uint32_t pos = atomicAdd(&cursor[(w << sp.CB) + (bi >> sp.FB)], 1u);
-                val1[pos] = (uint32_t)i | (d < 0 ? 0x80000000u : 0u);
-                key1[pos] = (uint16_t)(bi & fmask);
+                kv1[pos] = ((uint64_t)(bi & fmask) << 32) | (uint32_t)i | (d < 0 ? 0x80000000u : 0u);
             }
         }
     }
@@ -361,7 +382,7 @@ __global__ void __launch_bounds__(1024) bin_info_kernel(const uint32_t* __restri
     }
 }
 
-// largest b in [0, nbins) with base[b] <= t (base strictly increasing, base[0] = 0)
+// largest b in [0, nbins) with base[b] <= t (base nondecreasing, base[0] = 0)
 __device__ __forceinline__ uint32_t find_bin(const uint32_t* __restrict__ base, uint32_t nbins, uint32_t t)
 {
     uint32_t lo = 0, hi = nbins;
@@ -378,7 +399,7 @@ __device__ __forceinline__ uint32_t find_bin(const uint32_t* __restrict__ base, 
 // Round 2, pass 1: LDS histogram of the low bucket bits of one chunk of one
 // bin; stored at (bin, low, chunk) order. Blocks beyond the real chunk count
 // zero their slice so the scan over the upper bound stays exact.
-__global__ void __launch_bounds__(R2_THREADS) r2_count_kernel(const uint16_t* __restrict__ key1, SortParams sp,
+__global__ void __launch_bounds__(R2_THREADS) r2_count_kernel(const uint64_t* __restrict__ kv1, SortParams sp,
     const uint32_t* __restrict__ bin_start, const uint32_t* __restrict__ chunk_base, uint32_t* __restrict__ h2)
 {
     __shared__ uint32_t hist[1u << MAX_FB];
@@ -400,21 +421,26 @@ __global__ void __launch_bounds__(R2_THREADS) r2_count_kernel(const uint16_t* __
         end = bin_start[b + 1];
     __syncthreads();
     for (uint32_t e = start + threadIdx.x; e < end; e += R2_THREADS)
-        atomicAdd(&hist[key1[e]], 1u);
+        atomicAdd(&hist[(uint32_t)(kv1[e] >> 32)], 1u);
     __syncthreads();
     const uint64_t base = (uint64_t)chunk_base[b] << sp.FB;
     for (uint32_t low = threadIdx.x; low < FBN; low += R2_THREADS)
         h2[base + (uint64_t)low * kb + j] = hist[low];
 }
 
-// Round 2, pass 2: final placement; chunk 0 of each bin also publishes the
-// bucket offsets (bucket id = bin * 2^FB + low).
-__global__ void __launch_bounds__(R2_THREADS) r2_scatter_kernel(const uint16_t* __restrict__ key1,
-    const uint32_t* __restrict__ val1, SortParams sp, const uint32_t* __restrict__ bin_start,
-    const uint32_t* __restrict__ chunk_base, const uint32_t* __restrict__ h2off, uint32_t* __restrict__ entries,
-    uint32_t* __restrict__ off0)
+// Round 2, pass 2: the chunk is counting-sorted in LDS (ranks by LDS
+// atomics), then written out in LDS order, so consecutive threads store into
+// the same bucket run. Chunk 0 of each bin also publishes the bucket offsets
+// (bucket id = bin * 2^FB + low).
+__global__ void __launch_bounds__(R2_THREADS) r2_scatter_kernel(const uint64_t* __restrict__ kv1, SortParams sp,
+    const uint32_t* __restrict__ bin_start, const uint32_t* __restrict__ chunk_base,
+    const uint32_t* __restrict__ h2off, uint32_t* __restrict__ entries, uint32_t* __restrict__ off0)
 {
-    __shared__ uint32_t cursor[1u << MAX_FB];
+    __shared__ uint32_t cnt[1u << MAX_FB]; // histogram, then local cursor
+    __shared__ uint32_t loc[1u << MAX_FB]; // local exclusive offsets
+    __shared__ uint32_t part[R2_THREADS];
+    __shared__ uint32_t sval[R2_CHUNK];
+    __shared__ uint16_t skey[R2_CHUNK];
     const uint32_t FBN = 1u << sp.FB;
     const uint32_t t = blockIdx.x;
     if (t >= chunk_base[sp.NC])
@@ -423,121 +449,153 @@ __global__ void __launch_bounds__(R2_THREADS) r2_scatter_kernel(const uint16_t* 
     const uint32_t j = t - chunk_base[b];
     const uint32_t kb = chunk_base[b + 1] - chunk_base[b];
     const uint64_t base = (uint64_t)chunk_base[b] << sp.FB;
-    for (uint32_t low = threadIdx.x; low < FBN; low += R2_THREADS) {
-        const uint32_t o = h2off[base + (uint64_t)low * kb + j];
-        cursor[low] = o;
-        if (j == 0)
-            off0[((uint64_t)b << sp.FB) + low] = o;
-    }
     const uint32_t start = bin_start[b] + j * R2_CHUNK;
     uint32_t end = start + R2_CHUNK;
     if (end > bin_start[b + 1])
         end = bin_start[b + 1];
+    for (uint32_t low = threadIdx.x; low < FBN; low += R2_THREADS) {
+        cnt[low] = 0;
+        if (j == 0)
+            off0[((uint64_t)b << sp.FB) + low] = h2off[base + (uint64_t)low * kb];
+    }
+    __syncthreads();
+    for (uint32_t e = start + threadIdx.x; e < end; e += R2_THREADS)
+        atomicAdd(&cnt[(uint32_t)(kv1[e] >> 32)], 1u);
+    __syncthreads();
+    // exclusive scan of cnt into loc: FBN / R2_THREADS consecutive keys per thread
+    const uint32_t per = (FBN + R2_THREADS - 1) / R2_THREADS;
+    uint32_t sum = 0;
+    for (uint32_t q = 0; q < per; ++q) {
+        const uint32_t k = threadIdx.x * per + q;
+        if (k < FBN)
+            sum += cnt[k];
+    }
+    part[threadIdx.x] = sum;
+    for (uint32_t d = 1; d < R2_THREADS; d <<= 1) {
+        __syncthreads();
+        const uint32_t v = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+    }
+    __syncthreads();
+    uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+    for (uint32_t q = 0; q < per; ++q) {
+        const uint32_t k = threadIdx.x * per + q;
+        if (k < FBN) {
+            loc[k] = run;
+            run += cnt[k];
+            cnt[k] = loc[k];
+        }
+    }
     __syncthreads();
     for (uint32_t e = start + threadIdx.x; e < end; e += R2_THREADS) {
-        const uint32_t pos = atomicAdd(&cursor[key1[e]], 1u);
-        entries[pos] = val1[e];
+        const uint64_t kv = kv1[e];
+        const uint32_t low = (uint32_t)(kv >> 32);
+        const uint32_t r = atomicAdd(&cnt[low], 1u);
+        sval[r] = (uint32_t)kv;
+        skey[r] = (uint16_t)low;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < end - start; i += R2_THREADS) {
+        const uint32_t low = skey[i];
+        entries[h2off[base + (uint64_t)low * kb + j] + (i - loc[low])] = sval[i];
     }
 }
 
-__global__ void piece_count_kernel(const uint32_t* __restrict__ off, uint64_t nb, uint32_t S, uint32_t* __restrict__ pc)
+// Thread t adds the sorted entries [tS, min(tS + S, E)). A bucket wholly in
+// the range goes to bsum; the piece of a bucket that started before the
+// range goes to part[2t], the piece of one that runs past it to part[2t+1].
+// The next entry's point is fetched (raw words) before the current addition.
+__global__ void __launch_bounds__(256) accumulate_kernel(const uint32_t* __restrict__ entries,
+    const uint8_t* __restrict__ pts, const uint32_t* __restrict__ off, uint64_t nb, uint32_t S,
+    PointSlot* __restrict__ part, PointSlot* __restrict__ bsum)
 {
-    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb)
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t E = off[nb];
+    const uint64_t e0 = t * S;
+    if (e0 >= E)
         return;
-    const uint32_t cnt = off[b + 1] - off[b];
-    pc[b] = (cnt + S - 1) / S;
-}
-
-__global__ void piece_map_kernel(const uint32_t* __restrict__ poff, uint64_t nb, uint32_t* __restrict__ piece_bucket)
-{
-    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb)
-        return;
-    for (uint32_t p = poff[b]; p < poff[b + 1]; ++p)
-        piece_bucket[p] = (uint32_t)b;
-}
-
-// bases -> 2^261 scale (x * 2^261 = Montgomery product of x * 2^256 and 2^261);
-// the identity (0, 0) stays (0, 0)
-__global__ void to_scale261_kernel(const uint8_t* __restrict__ in, uint8_t* __restrict__ out, uint64_t n)
-{
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    Fq k;
-#pragma unroll
-    for (int q = 0; q < 8; ++q)
-        k.v[q] = Fq29Params::TO261[q];
-    (Fq::load(in + 64 * i) * k).store(out + 64 * i);
-    (Fq::load(in + 64 * i + 32) * k).store(out + 64 * i + 32);
-}
-
-// level 0: one thread per piece of <= S entries of one bucket; affine inputs.
-// A single-piece bucket is final (bsum); otherwise the partial goes to part.
-__global__ void __launch_bounds__(256) sum_pieces_kernel(const uint32_t* __restrict__ entries,
-    const uint8_t* __restrict__ points, const uint32_t* __restrict__ off, const uint32_t* __restrict__ poff,
-    const uint32_t* __restrict__ piece_bucket, uint64_t nb, uint32_t S, G1Xyzz* __restrict__ part,
-    G1Xyzz* __restrict__ bsum)
-{
-    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= poff[nb])
-        return;
-    const uint32_t b = piece_bucket[p];
-    const uint32_t first = poff[b];
-    const uint64_t start = off[b] + (uint64_t)(p - first) * S;
-    uint64_t end = start + S;
-    if (end > off[b + 1])
-        end = off[b + 1];
+    const uint32_t e1 = (uint32_t)(e0 + S < E ? e0 + S : E);
+    uint32_t b = find_bin(off, (uint32_t)nb, (uint32_t)e0); // the bucket holding e0 (empty ones skipped)
+    uint32_t bend = off[b + 1];
+    bool first = off[b] < e0; // the current bucket began before this range
     G1X29 acc = G1X29::inf();
-    for (uint64_t e = start; e < end; ++e) {
-        const uint32_t ent = entries[e];
-        G1A29 q = G1A29::load(points + 64ull * (ent & 0x7fffffffu));
+    uint32_t ent = entries[e0];
+    const uint4* src = (const uint4*)(pts + 64ull * (ent & 0x7fffffffu));
+    uint4 r0 = src[0], r1 = src[1], r2 = src[2], r3 = src[3];
+    for (uint32_t e = (uint32_t)e0; e < e1; ++e) {
+        const uint32_t en = entries[e + 1 < e1 ? e + 1 : e];
+        const uint4* sn = (const uint4*)(pts + 64ull * (en & 0x7fffffffu));
+        const uint4 n0 = sn[0], n1 = sn[1], n2 = sn[2], n3 = sn[3];
+        if (e == bend) {
+            // previous bucket complete within the range (or the range's first piece)
+            acc.store(first ? (void*)(part + 2 * t) : (void*)(bsum + b));
+            first = false;
+            acc = G1X29::inf();
+            do {
+                ++b;
+                bend = off[b + 1];
+            } while (bend == e);
+        }
+        const uint32_t xw[8] = { r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w };
+        const uint32_t yw[8] = { r2.x, r2.y, r2.z, r2.w, r3.x, r3.y, r3.z, r3.w };
+        G1A29 q { Fq29::from32(xw), Fq29::from32(yw) };
         if (ent >> 31)
             q = q.neg();
         acc = G1X29::add_affine(acc, q);
+        ent = en;
+        r0 = n0;
+        r1 = n1;
+        r2 = n2;
+        r3 = n3;
     }
-    if (poff[b + 1] - first == 1)
-        acc.store(bsum + b);
+    if (bend > e1)
+        acc.store(first ? (void*)(part + 2 * t) : (void*)(part + 2 * t + 1));
     else
-        acc.store(part + p);
+        acc.store(first ? (void*)(part + 2 * t) : (void*)(bsum + b));
 }
 
-// buckets with 2..LIGHT pieces: one thread sums them; more: queued for the
-// work-group reduction. heavy[nb] is the queue length (zeroed beforehand).
-__global__ void __launch_bounds__(256) merge_light_kernel(const G1Xyzz* __restrict__ part,
-    const uint32_t* __restrict__ poff, uint64_t nb, G1Xyzz* __restrict__ bsum, uint32_t* __restrict__ heavy)
+// A bucket split between threads t0 < t1 (t0 holds its first entry, so the
+// bucket did not begin before t0's range) has its pieces in part[2 t0 + 1]
+// and part[2 t] for t0 < t <= t1.
+// Up to LIGHT pieces: one thread sums them; more: queued (heavy[nb] counts).
+__global__ void __launch_bounds__(256) merge_split_kernel(const PointSlot* __restrict__ part,
+    const uint32_t* __restrict__ off, uint64_t nb, uint32_t S, PointSlot* __restrict__ bsum,
+    uint32_t* __restrict__ heavy)
 {
     const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb)
         return;
-    const uint32_t lo = poff[b], hi = poff[b + 1];
-    const uint32_t k = hi - lo;
-    if (k <= 1)
+    const uint32_t lo = off[b], hi = off[b + 1];
+    if (lo == hi)
         return;
-    if (k > LIGHT) {
+    const uint32_t t0 = lo / S, t1 = (hi - 1) / S;
+    if (t0 == t1)
+        return;
+    if (t1 - t0 + 1 > LIGHT) {
         const uint32_t slot = atomicAdd(&heavy[nb], 1u);
         heavy[slot] = (uint32_t)b;
         return;
     }
-    G1X29 acc = G1X29::load(part + lo);
-    for (uint32_t q = lo + 1; q < hi; ++q)
-        acc = G1X29::add(acc, G1X29::load(part + q));
+    G1X29 acc = G1X29::load(part + 2 * (uint64_t)t0 + 1);
+    for (uint32_t t = t0 + 1; t <= t1; ++t)
+        acc = G1X29::add(acc, G1X29::load(part + 2 * (uint64_t)t));
     acc.store(bsum + b);
 }
 
 // one work-group per queued dense bucket: strided partial sums, LDS tree
-__global__ void __launch_bounds__(HEAVY_THREADS) merge_heavy_kernel(const G1Xyzz* __restrict__ part,
-    const uint32_t* __restrict__ poff, uint64_t nb, const uint32_t* __restrict__ heavy, G1Xyzz* __restrict__ bsum)
+__global__ void __launch_bounds__(HEAVY_THREADS) merge_heavy_kernel(const PointSlot* __restrict__ part,
+    const uint32_t* __restrict__ off, uint64_t nb, uint32_t S, const uint32_t* __restrict__ heavy,
+    PointSlot* __restrict__ bsum)
 {
     __shared__ G1X29 sh[HEAVY_THREADS];
     const uint32_t count = heavy[nb];
     for (uint32_t h = blockIdx.x; h < count; h += gridDim.x) {
         const uint32_t b = heavy[h];
-        const uint32_t lo = poff[b], hi = poff[b + 1];
+        const uint32_t t0 = off[b] / S, t1 = (off[b + 1] - 1) / S;
         G1X29 acc = G1X29::inf();
-        for (uint32_t q = lo + threadIdx.x; q < hi; q += HEAVY_THREADS)
-            acc = G1X29::add(acc, G1X29::load(part + q));
+        for (uint32_t q = t0 + threadIdx.x; q <= t1; q += HEAVY_THREADS)
+            acc = G1X29::add(acc, G1X29::load(part + 2 * (uint64_t)q + (q == t0 ? 1 : 0)));
         sh[threadIdx.x] = acc;
         for (uint32_t d = HEAVY_THREADS / 2; d > 0; d >>= 1) {
             __syncthreads();
@@ -561,9 +619,9 @@ __device__ __forceinline__ G1X29 dbl_times(G1X29 x, uint32_t k)
 // first reduction level: segment m of window w covers buckets (1-based)
 // m*L+1 .. m*L+L.  T = sum_j j * B_{mL+j} (local weights), S = L * sum_j B.
 // An empty bucket (no entries) is the identity and its bsum slot is not read.
-__global__ void __launch_bounds__(256) reduce_first_kernel(const G1Xyzz* __restrict__ bsum,
+__global__ void __launch_bounds__(256) reduce_first_kernel(const PointSlot* __restrict__ bsum,
     const uint32_t* __restrict__ off, uint32_t W, uint32_t B, uint32_t L, uint32_t log_l, uint64_t M,
-    G1Xyzz* __restrict__ T, G1Xyzz* __restrict__ S)
+    PointSlot* __restrict__ T, PointSlot* __restrict__ S)
 {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (uint64_t)W * M)
@@ -582,39 +640,73 @@ __global__ void __launch_bounds__(256) reduce_first_kernel(const G1Xyzz* __restr
         dbl_times(run, log_l).store(S + t);
 }
 
-// later levels: answer = sum_m T_m + sum_m m * S_m over Mi segments; group g
-// of Lg segments gives T' = sum_i T_{gLg+i} + sum_i i * S_{gLg+i} and
-// S' = Lg * sum_i S_{gLg+i}, so the form is kept with Mi / Lg segments.
-__global__ void __launch_bounds__(256) reduce_level_kernel(const G1Xyzz* __restrict__ Ti,
-    const G1Xyzz* __restrict__ Si, uint32_t W, uint64_t Mi, uint32_t Lg, uint32_t log_lg, G1Xyzz* __restrict__ To,
-    G1Xyzz* __restrict__ So)
+// Tree level: work-group g of window w reduces the pairs
+// [g * cnt * K, (g + 1) * cnt * K) of the Mi pairs of that window
+// (answer = sum T + sum_m m S_m, S in units of the current segment size).
+// Leaf i folds K consecutive pairs with running sums (local weights 0..K-1,
+// S scaled by K); adjacent blocks of equal size then combine as
+// T = T_L + T_R + S_R, S = 2 (S_L + S_R); after log2(cnt) rounds the
+// block's (T, S) is in the same form with cnt * K times larger segments.
+__global__ void __launch_bounds__(TREE) reduce_tree_kernel(const PointSlot* __restrict__ Ti,
+    const PointSlot* __restrict__ Si, uint64_t Mi, uint32_t K, uint32_t log_k, PointSlot* __restrict__ To,
+    PointSlot* __restrict__ So)
 {
-    const uint64_t Mo = Mi / Lg;
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (uint64_t)W * Mo)
-        return;
-    const uint64_t w = t / Mo, g = t % Mo;
-    const uint64_t base = w * Mi + g * Lg;
-    G1X29 a = G1X29::inf(), run = G1X29::inf(), acc = G1X29::inf();
-    for (int i = (int)Lg - 1; i >= 0; --i) {
-        a = G1X29::add(a, G1X29::load(Ti + base + i));
-        run = G1X29::add(run, G1X29::load(Si + base + i));
-        if (i > 0)
-            acc = G1X29::add(acc, run);
+    __shared__ G1X29 sT[TREE];
+    __shared__ G1X29 sS[TREE];
+    const uint64_t per_blk = Mi >= (uint64_t)TREE * K ? (uint64_t)TREE * K : Mi;
+    const uint64_t Mo = Mi / per_blk;
+    const uint32_t cnt = (uint32_t)(per_blk / K);
+    const uint64_t w = blockIdx.x / Mo, g = blockIdx.x % Mo;
+    const uint64_t base = w * Mi + g * per_blk;
+    const uint32_t i = threadIdx.x;
+    const bool keep_s = cnt > 1 || Mo > 1;
+    if (i < cnt) {
+        G1X29 a = G1X29::inf(), run = G1X29::inf(), acc = G1X29::inf();
+        for (int j = (int)K - 1; j >= 0; --j) {
+            const uint64_t q = base + (uint64_t)i * K + (uint32_t)j;
+            a = G1X29::add(a, G1X29::load(Ti + q));
+            run = G1X29::add(run, G1X29::load(Si + q));
+            if (j > 0)
+                acc = G1X29::add(acc, run);
+        }
+        sT[i] = G1X29::add(a, acc);
+        if (keep_s)
+            sS[i] = dbl_times(run, log_k);
     }
-    G1X29::add(a, acc).store(To + t);
-    if (Mo > 1)
-        dbl_times(run, log_lg).store(So + t);
+    for (uint32_t d = 1; d < cnt; d <<= 1) {
+        __syncthreads();
+        G1X29 t, s;
+        const bool act = (i & (2 * d - 1)) == 0 && i + d < cnt;
+        const bool need_s = 2 * d < cnt || Mo > 1;
+        if (act) {
+            t = G1X29::add(G1X29::add(sT[i], sT[i + d]), sS[i + d]);
+            if (need_s)
+                s = G1X29::dbl(G1X29::add(sS[i], sS[i + d]));
+        }
+        __syncthreads();
+        if (act) {
+            sT[i] = t;
+            if (need_s)
+                sS[i] = s;
+        }
+    }
+    __syncthreads();
+    if (i == 0) {
+        sT[0].store(To + blockIdx.x);
+        if (Mo > 1)
+            sS[0].store(So + blockIdx.x);
+    }
 }
 
-__global__ void __launch_bounds__(64) finalize_kernel(const G1Xyzz* __restrict__ win, uint32_t W, uint32_t c, uint64_t* __restrict__ out)
+__global__ void __launch_bounds__(64) finalize_kernel(const PointSlot* __restrict__ win, uint32_t W, uint32_t c,
+    uint64_t* __restrict__ out)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0)
         return;
     G1X29 acc = G1X29::load(win + (W - 1));
     for (int w = (int)W - 2; w >= 0; --w) {
         for (uint32_t k = 0; k < c; ++k)
-            acc = G1X29::dbl(acc);
+            acc = G1X29::dbl_inline(acc);
         acc = G1X29::add(acc, G1X29::load(win + w));
     }
     Fq one = Fq::one();
@@ -652,36 +744,29 @@ RustError msm_device(const MsmPlan& p, const MsmBuffers& b, const Fr* d_scalars,
     RustError e = scan_u32_exclusive(b.h1, b.h1off, nh1, b.scan_scratch, stream);
     if (e.code)
         return e;
-    r1_scatter_kernel<<<p.ntiles, R1_THREADS, 4 * p.NC, stream>>>(b.canon, sp, b.h1off, b.val1, b.key1);
+    r1_scatter_kernel<<<p.ntiles, R1_THREADS, 4 * p.NC, stream>>>(b.canon, sp, b.h1off, b.kv1);
     H2_LAUNCH_CHECK();
     bin_info_kernel<<<1, 1024, 0, stream>>>(b.h1off, sp, b.bin_start, b.chunk_base, b.off0 + p.nb);
     H2_LAUNCH_CHECK();
-    r2_count_kernel<<<(unsigned)p.max_chunks, R2_THREADS, 0, stream>>>(b.key1, sp, b.bin_start, b.chunk_base, b.h2);
+    r2_count_kernel<<<(unsigned)p.max_chunks, R2_THREADS, 0, stream>>>(b.kv1, sp, b.bin_start, b.chunk_base, b.h2);
     H2_LAUNCH_CHECK();
     e = scan_u32_exclusive(b.h2, b.h2off, p.max_chunks << p.FB, b.scan_scratch, stream);
     if (e.code)
         return e;
-    r2_scatter_kernel<<<(unsigned)p.max_chunks, R2_THREADS, 0, stream>>>(b.key1, b.val1, sp, b.bin_start,
-        b.chunk_base, b.h2off, b.entries, b.off0);
+    r2_scatter_kernel<<<(unsigned)p.max_chunks, R2_THREADS, 0, stream>>>(b.kv1, sp, b.bin_start, b.chunk_base,
+        b.h2off, b.entries, b.off0);
     H2_LAUNCH_CHECK();
     prof_sort.reset();
 
     {
         ProfScope prof("msm_buckets", stream);
-        piece_count_kernel<<<blocks_for(p.nb, T), T, 0, stream>>>(b.off0, p.nb, p.S0, b.pc);
-        H2_LAUNCH_CHECK();
-        e = scan_u32_exclusive(b.pc, b.poff, p.nb, b.scan_scratch, stream);
-        if (e.code)
-            return e;
-        piece_map_kernel<<<blocks_for(p.nb, T), T, 0, stream>>>(b.poff, p.nb, b.piece_bucket);
-        H2_LAUNCH_CHECK();
-        sum_pieces_kernel<<<blocks_for(p.ub0, T), T, 0, stream>>>(b.entries, b.pts, b.off0, b.poff,
-            b.piece_bucket, p.nb, p.S0, b.part, b.bsum);
+        accumulate_kernel<<<blocks_for(p.nthr, T), T, 0, stream>>>(b.entries, b.pts, b.off0, p.nb, p.S, b.part,
+            b.bsum);
         H2_LAUNCH_CHECK();
         H2_TRY(hipMemsetAsync(b.heavy + p.nb, 0, 4, stream));
-        merge_light_kernel<<<blocks_for(p.nb, T), T, 0, stream>>>(b.part, b.poff, p.nb, b.bsum, b.heavy);
+        merge_split_kernel<<<blocks_for(p.nb, T), T, 0, stream>>>(b.part, b.off0, p.nb, p.S, b.bsum, b.heavy);
         H2_LAUNCH_CHECK();
-        merge_heavy_kernel<<<HEAVY_BLOCKS, HEAVY_THREADS, 0, stream>>>(b.part, b.poff, p.nb, b.heavy, b.bsum);
+        merge_heavy_kernel<<<HEAVY_BLOCKS, HEAVY_THREADS, 0, stream>>>(b.part, b.off0, p.nb, p.S, b.heavy, b.bsum);
         H2_LAUNCH_CHECK();
     }
 
@@ -693,9 +778,9 @@ RustError msm_device(const MsmPlan& p, const MsmBuffers& b, const Fr* d_scalars,
     int cur = 0;
     for (uint32_t lev = 1; lev <= p.nlev; ++lev) {
         const uint64_t Mi = p.Mlev[lev - 1];
-        const uint32_t lg = p.Llev[lev];
-        reduce_level_kernel<<<blocks_for((uint64_t)p.W * (Mi / lg), T), T, 0, stream>>>(b.red[2 * cur],
-            b.red[2 * cur + 1], p.W, Mi, lg, ilog2(lg), b.red[2 * (cur ^ 1)], b.red[2 * (cur ^ 1) + 1]);
+        const uint64_t Mo = p.Mlev[lev];
+        reduce_tree_kernel<<<(unsigned)(p.W * Mo), TREE, 0, stream>>>(b.red[2 * cur], b.red[2 * cur + 1], Mi,
+            p.Klev[lev], ilog2(p.Klev[lev]), b.red[2 * (cur ^ 1)], b.red[2 * (cur ^ 1) + 1]);
         H2_LAUNCH_CHECK();
         cur ^= 1;
     }

@@ -277,25 +277,53 @@ __global__ void __launch_bounds__(1024) ntt_pass_kernel(PassBatch pb, PassParams
         lds_put(lds, RG, slot, x);
     }
 
-    const uint32_t halfR = R >> 1;
-    for (uint32_t s = 0; s < log_r; ++s) {
+    // Stages go in pairs (radix 4: four elements per thread through LDS once
+    // for two stages); an odd stage count starts with one radix-2 stage 0,
+    // which has only unit twiddles. Per element the operations are the same
+    // as radix-2, so the bounds above hold.
+    uint32_t s = 0;
+    if (log_r & 1u) {
         __syncthreads();
-        const uint32_t half = 1u << s;
         for (uint32_t q = threadIdx.x; q < (RG >> 1); q += T) {
-            const uint32_t g = q >> (log_r - 1);
-            const uint32_t qq = q & (halfR - 1);
-            const uint32_t pos = qq & (half - 1);
-            const uint32_t blk = qq >> s;
-            const uint32_t i0 = (g << log_r) + (blk << (s + 1)) + pos;
-            const uint32_t i1 = i0 + half;
-            Fr29 a = lds_get(lds, RG, i0);
-            Fr29 c = lds_get(lds, RG, i1);
-            if (s) {
-                a = Fr29::normalize(a);
-                c = Fr29::mul(c, Fr29::load32(pp.twr + (pos << (log_r - 1 - s))));
-            }
+            const uint32_t i0 = q << 1;
+            const Fr29 a = lds_get(lds, RG, i0);
+            const Fr29 c = lds_get(lds, RG, i0 + 1);
             lds_put(lds, RG, i0, Fr29::add(a, c));
-            lds_put(lds, RG, i1, Fr29::sub2(a, c));
+            lds_put(lds, RG, i0 + 1, Fr29::sub2(a, c));
+        }
+        s = 1;
+    }
+    const uint32_t quarterR = R >> 2;
+    for (; s + 1 < log_r; s += 2) {
+        __syncthreads();
+        const uint32_t h = 1u << s;
+        for (uint32_t q = threadIdx.x; q < (RG >> 2); q += T) {
+            const uint32_t g = q >> (log_r - 2);
+            const uint32_t qq = q & (quarterR - 1);
+            const uint32_t pos = qq & (h - 1);
+            const uint32_t blk = qq >> s;
+            const uint32_t i0 = (g << log_r) + (blk << (s + 2)) + pos;
+            Fr29 x0 = lds_get(lds, RG, i0);
+            Fr29 x1 = lds_get(lds, RG, i0 + h);
+            Fr29 x2 = lds_get(lds, RG, i0 + 2 * h);
+            Fr29 x3 = lds_get(lds, RG, i0 + 3 * h);
+            if (s) {
+                const Fr29 w = Fr29::load32(pp.twr + (pos << (log_r - 1 - s)));
+                x0 = Fr29::normalize(x0);
+                x2 = Fr29::normalize(x2);
+                x1 = Fr29::mul(x1, w);
+                x3 = Fr29::mul(x3, w);
+            }
+            Fr29 y0 = Fr29::add(x0, x1), y1 = Fr29::sub2(x0, x1);
+            Fr29 y2 = Fr29::add(x2, x3), y3 = Fr29::sub2(x2, x3);
+            y0 = Fr29::normalize(y0);
+            y1 = Fr29::normalize(y1);
+            y2 = Fr29::mul(y2, Fr29::load32(pp.twr + (pos << (log_r - 2 - s))));
+            y3 = Fr29::mul(y3, Fr29::load32(pp.twr + ((pos + h) << (log_r - 2 - s))));
+            lds_put(lds, RG, i0, Fr29::add(y0, y2));
+            lds_put(lds, RG, i0 + 2 * h, Fr29::sub2(y0, y2));
+            lds_put(lds, RG, i0 + h, Fr29::add(y1, y3));
+            lds_put(lds, RG, i0 + 3 * h, Fr29::sub2(y1, y3));
         }
     }
     __syncthreads();

@@ -151,6 +151,29 @@ def test_msm_closed_form_large(lib, co, k):
     assert (got == exp).all()
 
 
+def test_msm_closed_form_dense_2_20(lib, co):
+    """Every scalar equal (one bucket per window holding all 2^20 entries, merged by the
+    work-group tree) and a half-zero / half-constant mix: closed form (c sum s_i) G."""
+    import torch
+
+    from halo2_amd import device as dv, msm
+
+    n = 1 << 20
+    print("msm dense closed form n=2^20", flush=True)
+    ds = msm.random_fr(n, 71)
+    pts = msm.g1_mul_generator(ds, n)
+    k = _dense_constant()
+    dc = torch.tensor(np.tile(k, n).view(np.int64), device=ds.device)
+    got = msm.multiexp_device_scalars_device_bases(dc, pts, n)
+    hs = dv.to_host(ds)
+    exp = co.g1_mul(co.g1_generator(), co.fr_inner_product(np.tile(k, (n, 1)), hs))
+    assert (got == exp).all()
+    half = np.tile(k, (n, 1))
+    half[::2] = 0
+    got = msm.multiexp_device_scalars_device_bases(dv.to_device(half), pts, n)
+    assert (got == co.g1_mul(co.g1_generator(), co.fr_inner_product(half, hs))).all()
+
+
 def test_kzg_commit_srs_identity(lib, co):
     """ParamsKZG with a seeded tau: commit(f) = f(tau) G, resident SRS, device and host polys."""
     from halo2_amd import device as dv, fields as F, msm
