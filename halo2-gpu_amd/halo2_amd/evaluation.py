@@ -1,0 +1,123 @@
+"""h(X) on the extended domain, part by part, on the device: the orchestration of
+`evaluate_h_device` / `evaluate_h_inner` (src/plonk/evaluation.rs:1056-1417 of the
+reference; the CPU form it must agree with is src/cpu/evaluator.rs:150-411).
+
+For each of the 2^(extended_k - k) parts (coset shift zeta * extended_omega^p):
+  1. every fixed / selector / advice / instance polynomial is moved to the part
+     (`coeff_to_extended_part`),
+  2. custom gates: the rule interpreter over the part's columns (`_halo2_quotient*`),
+  3. permutation identities (`_halo2_quotient_permutation`) over the cosets of the
+     grand products and of the sigma polynomials,
+  4. per lookup: the compressed (input + beta)(table + gamma) values through the rule
+     interpreter with runtime constants, then `_halo2_quotient_lookups` over the cosets
+     of the product and permuted polynomials,
+then the parts are interleaved into the extended Lagrange vector
+(`_halo2_extended_from_lagrange_vec_device`). One circuit (the reference folds a batch
+of circuits through the same loop).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import device as dv
+from .fields import DELTA, R_MOD, ZETA, mont_limbs
+from .quotient import ColumnPool, GraphEvaluator, compress_expressions_with_runtime_constants_device, \
+    quotient_device
+
+
+@dataclass
+class LookupCommitted:
+    """lookup::prover::Committed: the product polynomial (coefficients) and the permuted
+    input / table expressions (Lagrange values), plus the lookup's evaluator graph."""
+
+    graph: GraphEvaluator
+    product_poly: torch.Tensor
+    permuted_input: torch.Tensor
+    permuted_table: torch.Tensor
+
+
+@dataclass
+class PermutationCommitted:
+    """permutation::prover::Committed: one grand-product polynomial (coefficients) per set,
+    the columns of the argument as (kind, index) with kind in fixed/advice/instance, and
+    the proving key's sigma polynomials (coefficients), one per column."""
+
+    product_polys: list
+    columns: list
+    sigma_polys: list = field(default_factory=list)
+
+
+def _host_scalar(v: int) -> np.ndarray:
+    return np.ascontiguousarray(mont_limbs(v % R_MOD))
+
+
+def _ptr_table(tensors) -> torch.Tensor:
+    return torch.tensor([dv.ptr(t) for t in tensors] or [0], dtype=torch.int64, device=dv.device())
+
+
+def lagrange_to_extend_part(domain, poly: torch.Tensor, extended_omega_factor: int) -> torch.Tensor:
+    """domain.rs:281: Lagrange values -> coefficients -> the coset part."""
+    coeff = domain.lagrange_to_coeff_device_input(poly)
+    return domain.coeff_to_extended_part_many_device([coeff], extended_omega_factor)[0]
+
+
+def evaluate_h_device(domain, gates: GraphEvaluator, fixed_polys, l0, l_last, l_active, advice_polys,
+                      instance_polys, challenges, y: int, beta: int, gamma: int, theta: int,
+                      lookups: list[LookupCommitted] = (), permutation: PermutationCommitted | None = None,
+                      blinding_factors: int = 0, cs_degree: int = 3) -> torch.Tensor:
+    """Extended Lagrange values of h's numerator (before the vanishing division), one
+    circuit; all polynomials are device tensors of n coefficients (permuted lookup
+    columns: n Lagrange values)."""
+    lib = _lib.load()
+    n = domain.n
+    num_parts = domain.extended_len >> domain.k
+    stream = dv.stream()
+    cur = 1
+    nf, na = len(fixed_polys), len(advice_polys)
+    value_parts = []
+    for _ in range(num_parts):
+        polys = list(fixed_polys) + [l0, l_last, l_active] + list(advice_polys) + list(instance_polys)
+        parts = domain.coeff_to_extended_part_many_device(polys, cur)
+        fixed = parts[:nf]
+        l0_e, ll_e, la_e = parts[nf:nf + 3]
+        advice = parts[nf + 3:nf + 3 + na]
+        instance = parts[nf + 3 + na:]
+        pool = ColumnPool(n)
+        pool.try_init_device(fixed, [], advice, instance)
+
+        values = quotient_device(gates, fixed, instance, advice, challenges, y, n, 1)
+
+        if permutation is not None and permutation.product_polys:
+            chunk_len = cs_degree - 2
+            last_rotation = -(blinding_factors + 1)
+            zs = domain.coeff_to_extended_part_many_device(permutation.product_polys, cur)
+            sig = domain.coeff_to_extended_part_many_device(permutation.sigma_polys, cur)
+            groups = {"fixed": fixed, "advice": advice, "instance": instance}
+            cols = [groups[kind][idx] for kind, idx in permutation.columns]
+            tz, ts, tc = _ptr_table(zs), _ptr_table(sig), _ptr_table(cols)
+            sc = [_host_scalar(v) for v in (beta, gamma, y, DELTA, beta * ZETA, cur, domain.omega)]
+            _lib.check(lib._halo2_quotient_permutation(
+                dv.ptr(values), dv.ptr(l0_e), dv.ptr(ll_e), dv.ptr(la_e), dv.ptr(tz), dv.ptr(ts), dv.ptr(tc),
+                *[ctypes.c_void_p(x.ctypes.data) for x in sc], len(zs), chunk_len, len(cols), last_rotation, 1, n,
+                n, stream), "_halo2_quotient_permutation")
+
+        if lookups:
+            table = dv.empty_u64(4 * n)
+            d_sc = [dv.to_device(mont_limbs(v % R_MOD).reshape(1, 4)) for v in (beta, gamma, y)]
+            for lk in lookups:
+                compress_expressions_with_runtime_constants_device(lk.graph, theta, beta, gamma, y, n, 1, pool,
+                                                                   challenges, table)
+                pin = lagrange_to_extend_part(domain, lk.permuted_input, cur)
+                ptab = lagrange_to_extend_part(domain, lk.permuted_table, cur)
+                prod = domain.coeff_to_extended_part_many_device([lk.product_poly], cur)[0]
+                _lib.check(lib._halo2_quotient_lookups(
+                    dv.ptr(values), dv.ptr(table), dv.ptr(prod), dv.ptr(pin), dv.ptr(ptab), dv.ptr(l0_e),
+                    dv.ptr(ll_e), dv.ptr(la_e), *[dv.ptr(x) for x in d_sc], n, stream), "_halo2_quotient_lookups")
+        value_parts.append(values)
+        cur = cur * domain.extended_omega % R_MOD
+    return domain.extended_from_lagrange_vec_device(value_parts)

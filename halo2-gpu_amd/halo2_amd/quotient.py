@@ -207,6 +207,52 @@ class GraphEvaluator:
             raise ValueError(op)
         return a | (c1 << 40) | (d << 44), b | (c2 << 40)
 
+    def encode_for_device_with_runtime_constants(self, theta: int, beta: int, gamma: int, y: int):
+        """evaluation.rs:695-815: (constants + [theta, beta, gamma, y], rules, value-part rules).
+        Beta/Gamma/Theta/Y sources become Constant slots, internal Horners are unrolled into
+        Mul/Add chains (intermediates renumbered), a non-Horner terminal becomes the single
+        part of Horner(Constant(0), [term], Constant(0))."""
+        constants = list(self.constants)
+        slot = {}
+        for name, v in (("Theta", theta), ("Beta", beta), ("Gamma", gamma), ("Y", y)):
+            slot[name] = len(constants)
+            constants.append(v % R_MOD)
+        remap: list[int] = []
+        new: list[tuple] = []
+
+        def rw(v):
+            if v[1] in slot:
+                return _vs("Constant", slot[v[1]])
+            if v[1] == "Intermediate":
+                return _vs("Intermediate", remap[v[2]])
+            return v
+
+        *internal, (terminal, _) = self.calculations
+        for calc, _target in internal:
+            if calc[0] == "Horner":
+                start, parts, factor = rw(calc[1]), calc[2], rw(calc[3])
+                if not parts:
+                    new.append(("Store", start))
+                    remap.append(len(new) - 1)
+                    continue
+                acc = start
+                for part in parts:
+                    new.append(("Mul", acc, factor))
+                    new.append(("Add", _vs("Intermediate", len(new) - 1), rw(part)))
+                    acc = _vs("Intermediate", len(new) - 1)
+                remap.append(len(new) - 1)
+            else:
+                new.append((calc[0],) + tuple(rw(a) for a in calc[1:]))
+                remap.append(len(new) - 1)
+        if terminal[0] == "Horner":
+            parts = tuple(rw(p) for p in terminal[2])
+        else:
+            new.append((terminal[0],) + tuple(rw(a) for a in terminal[1:]))
+            parts = (_vs("Intermediate", len(new) - 1),)
+        rules = np.array([self._encode_calc(c) for c in new], dtype=np.uint64).reshape(-1, 2)
+        vp = np.array([self._encode_vs(p) for p in parts], dtype=np.uint64)
+        return constants, rules, vp
+
     def encode_for_device(self) -> tuple[np.ndarray, np.ndarray]:
         """(rules: (n, 2) uint64, value_part_rules: (m,) uint64); the last calculation must be
         the terminal Horner (evaluation.rs:641-650)."""
@@ -215,6 +261,23 @@ class GraphEvaluator:
         rules = np.array([self._encode_calc(c) for c, _ in body], dtype=np.uint64).reshape(-1, 2)
         vp = np.array([self._encode_vs(p) for p in horner[2]], dtype=np.uint64)
         return rules, vp
+
+
+def lookup_evaluator(input_expressions, table_expressions) -> GraphEvaluator:
+    """Evaluator::new_inner, lookup part (evaluation.rs:417-446): theta-compressed input and
+    table, then (input + beta) * (table + gamma)."""
+    g = GraphEvaluator()
+
+    def lc(exprs):
+        parts = tuple(g.add_expression(e) for e in exprs)
+        return g.add_calculation(("Horner", _vs("Constant", 0), parts, _vs("Theta")))
+
+    ci = lc(input_expressions)
+    ct = lc(table_expressions)
+    right = g.add_calculation(("Add", ct, _vs("Gamma")))
+    left = g.add_calculation(("Add", ci, _vs("Beta")))
+    g.add_calculation(("Mul", left, right))
+    return g
 
 
 def custom_gates_evaluator(gate_polys) -> GraphEvaluator:
@@ -437,23 +500,27 @@ def _pool_chunks(lib, rules, nvp: int, size: int, pool: ColumnPool, n_chal: int,
 
 
 def _launch_pool(lib, graph: GraphEvaluator, expr_constants, size: int, rot_scale: int, pool: ColumnPool,
-                 challenges, out_host: np.ndarray | None, out_dev: torch.Tensor | None) -> None:
+                 challenges, out_host: np.ndarray | None, out_dev: torch.Tensor | None, encoded=None) -> None:
     assert pool.is_initialized(), "ColumnPool is not initialised"
-    rules, vp = graph.encode_for_device()
+    if encoded is None:
+        rules, vp = graph.encode_for_device()
+        constants = graph.constants
+    else:
+        constants, rules, vp = encoded
     rules = np.ascontiguousarray(rules)
     nvp = len(vp)
     assert nvp > 0
     vp = np.ascontiguousarray(vp)
-    consts = _fr_host(graph.constants)
+    consts = _fr_host(constants)
     chals = _fr_host(challenges)
     expr = _fr_host(expr_constants)
     heads = (pool.fixed_ptrs(), pool.num_fixed(), pool.instance_ptrs(), pool.num_instance(), pool.advice_ptrs(),
              pool.num_advice(), _lib.obj(chals.ctypes.data), len(challenges), _lib.obj(consts.ctypes.data),
-             len(graph.constants), _lib.obj(expr.ctypes.data), rules.ctypes.data, len(rules), vp.ctypes.data, nvp)
-    for r0, r1 in _pool_chunks(lib, rules, nvp, size, pool, len(challenges), len(graph.constants), rot_scale):
+             len(constants), _lib.obj(expr.ctypes.data), rules.ctypes.data, len(rules), vp.ctypes.data, nvp)
+    for r0, r1 in _pool_chunks(lib, rules, nvp, size, pool, len(challenges), len(constants), rot_scale):
         ws = int(lib._halo2_quotient_workspace_size(rules.ctypes.data, len(rules), nvp, size, pool.num_fixed(),
                                                     pool.num_instance(), pool.num_advice(), len(challenges),
-                                                    len(graph.constants), rot_scale, r0, r1))
+                                                    len(constants), rot_scale, r0, r1))
         scratch = dv.empty_bytes(ws)
         if out_host is not None:
             chunk = out_host[r0:r1]
@@ -484,3 +551,15 @@ def compress_expressions_in_place_device(graph: GraphEvaluator, expr_constants, 
     lib = _lib.load()
     assert d_out.numel() * 8 >= 32 * size
     _launch_pool(lib, graph, expr_constants, size, rot_scale, pool, challenges, None, d_out)
+
+
+def compress_expressions_with_runtime_constants_device(graph: GraphEvaluator, theta: int, beta: int, gamma: int,
+                                                       y: int, size: int, rot_scale: int, pool: ColumnPool,
+                                                       challenges, d_out: torch.Tensor) -> None:
+    """evaluation.rs:1856-1990: a graph that reads Beta/Gamma/Theta/Y (the lookup evaluator),
+    flattened with those values as trailing constants; expression slot 4 is 0, so each row's
+    value is the terminal term itself. Device output in `d_out` (size rows)."""
+    lib = _lib.load()
+    assert d_out.numel() * 8 >= 32 * size
+    enc = graph.encode_for_device_with_runtime_constants(theta, beta, gamma, y)
+    _launch_pool(lib, graph, [0, 1, R_MOD - 1, 2, 0], size, rot_scale, pool, challenges, None, d_out, encoded=enc)
