@@ -247,6 +247,35 @@ struct F29 {
         return z == 0;
     }
 
+    // a^(m - 2) with Montgomery products (2-bit fixed windows, a small register
+    // footprint: it runs in one thread of kernels that keep many values live): for
+    // a = u * 2^261 this is u^-1 * 2^261; zero maps to zero. Input normalised, < 2m.
+    __device__ static F29 pow_m_minus_2(const F29& a)
+    {
+        const F29 a2 = mul(a, a);
+        const F29 a3 = mul(a2, a);
+        uint32_t e[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            e[i] = P::Base::M[i];
+        e[0] -= 2u; // m is odd and > 2, no borrow
+        F29 r = one();
+        bool started = false;
+        for (int w = 127; w >= 0; --w) {
+            const uint32_t d = (e[w >> 4] >> ((w & 15) * 2)) & 3u;
+            if (started) {
+                r = mul(r, r);
+                r = mul(r, r);
+            }
+            if (d) {
+                const F29& s = d == 1 ? a : (d == 2 ? a2 : a3);
+                r = started ? mul(r, s) : s;
+                started = true;
+            }
+        }
+        return r;
+    }
+
     // normalised input of value < 2^261 -> canonical representative in [0, m)
     __device__ __forceinline__ static F29 reduce_full(const F29& a)
     {

@@ -15,6 +15,7 @@
 // scratch memory is needed, matching the scratch-free device entry point.
 #include "common.h"
 #include "field.h"
+#include "field29.h"
 
 using namespace h2;
 
@@ -48,9 +49,14 @@ __device__ void block_scan_inclusive(F* s, unsigned tid, bool reverse)
     __syncthreads();
 }
 
-template <class F>
-__global__ void __launch_bounds__(T) batch_invert_kernel(F* __restrict__ data, uint64_t n)
+// Batch inversion in the 29-bit-limb form (field29.h). Stored words X = x * 2^256
+// are read as 2^261-scale representatives of u = X / 2^261; the trick computes
+// u^-1 * 2^261 = 2^522 / X, and one factor 2^-10 (a product with 2^251) applied to
+// the work-group's inverse turns every output into 2^512 / X = (x^-1) * 2^256.
+template <class P>
+__global__ void __launch_bounds__(T) batch_invert_kernel(uint8_t* __restrict__ data, uint64_t n)
 {
+    using F = F29<P>;
     __shared__ F pre[T];
     __shared__ F suf[T];
     __shared__ F inv_total;
@@ -63,7 +69,7 @@ __global__ void __launch_bounds__(T) batch_invert_kernel(F* __restrict__ data, u
         const uint64_t i = base + (uint64_t)k * T + tid;
         part[k] = acc;
         if (i < n) {
-            F a = F::load(data + i);
+            const F a = F::load32(data + 32 * i);
             if (!a.is_zero())
                 acc = F::mul(acc, a);
         }
@@ -72,8 +78,11 @@ __global__ void __launch_bounds__(T) batch_invert_kernel(F* __restrict__ data, u
     suf[tid] = acc;
     block_scan_inclusive(pre, tid, false);
     block_scan_inclusive(suf, tid, true);
-    if (tid == 0)
-        inv_total = F::inverse(pre[T - 1]);
+    if (tid == 0) {
+        F k251 = F::zero();
+        k251.l[8] = 1u << 19; // 2^251 = 2^(232 + 19)
+        inv_total = F::mul(F::pow_m_minus_2(pre[T - 1]), k251);
+    }
     __syncthreads();
     // inverse of this thread's product = (prod of others before) * (after) / total
     F inv = inv_total;
@@ -85,14 +94,25 @@ __global__ void __launch_bounds__(T) batch_invert_kernel(F* __restrict__ data, u
     for (int k = (int)E - 1; k >= 0; --k) {
         const uint64_t i = base + (uint64_t)k * T + tid;
         if (i < n) {
-            F a = F::load(data + i);
+            const F a = F::load32(data + 32 * i);
             if (!a.is_zero()) {
-                F::mul(inv, part[k]).store(data + i);
+                F::reduce_full(F::mul(inv, part[k])).store32(data + 32 * i);
                 inv = F::mul(inv, a);
             }
         }
     }
 }
+
+template <class F>
+struct Form29;
+template <>
+struct Form29<Fr> {
+    using P = Fr29P;
+};
+template <>
+struct Form29<Fq> {
+    using P = Fq29P;
+};
 
 template <class F>
 RustError batch_invert(F* data, uint64_t n, hipStream_t stream)
@@ -101,7 +121,7 @@ RustError batch_invert(F* data, uint64_t n, hipStream_t stream)
         return ok();
     const uint64_t blocks = (n + C - 1) / C;
     H2_CHECK_ARG(blocks < (1ull << 31), "batch invert of %lu elements is too large", (unsigned long)n);
-    batch_invert_kernel<F><<<(unsigned)blocks, T, 0, stream>>>(data, n);
+    batch_invert_kernel<typename Form29<F>::P><<<(unsigned)blocks, T, 0, stream>>>((uint8_t*)data, n);
     H2_LAUNCH_CHECK();
     return ok();
 }
