@@ -27,6 +27,13 @@ class FFITraitObject(ctypes.Structure):
     _fields_ = [("ptr", ctypes.c_void_p)]
 
 
+class AssignedLayout(ctypes.Structure):
+    """src/cuda/culib.rs:20-26: byte layout of one Assigned<F> element."""
+
+    _fields_ = [("stride_bytes", ctypes.c_uint32), ("num_offset", ctypes.c_uint32),
+                ("denom_offset", ctypes.c_uint32)]
+
+
 class Halo2GpuError(RuntimeError):
     def __init__(self, code: int, message: str):
         super().__init__(f"[{code}] {message}")
@@ -102,6 +109,30 @@ SIGNATURES = {
                                                               P, U64, P, U64, U64, U64, P, U64, P]),
     "_halo2_generate_omega_powers_workspace_size": (U64, [U32]),
     "_halo2_generate_omega_powers": (RustError, [P, P, U32, P, U64, P]),
+    "_halo2_power_of_omega_workspace_size": (U64, [U32]),
+    "_halo2_power_of_omega": (RustError, [OBJP, OBJP, OBJP, U32, U32, P, U64, P]),
+    "_halo2_generate_omega_lut_workspace_size": (U64, [U32]),
+    "_halo2_generate_omega_lut": (RustError, [OBJP, OBJP, U32, P, U64, P]),
+    "_halo2_generate_omegadelta_workspace_size": (U64, [U32, U32, U32, U32, U32]),
+    "_halo2_generate_omegadelta": (RustError, [P, P, P, P, U32, U32, U32, U32, U32, P, U64, P]),
+    "_halo2_eval_polynomial_workspace_size": (U64, [U64]),
+    "_halo2_eval_polynomial": (RustError, [P, OBJP, ctypes.POINTER(P), U64, P, U64, P]),
+    "_halo2_eval_poly_batch_max_len": (SZ, [SZ, SZ, U64]),
+    "_halo2_eval_polynomial_batch_workspace_size": (U64, [U64, U64]),
+    "_halo2_eval_polynomial_batch": (RustError, [OBJP, OBJP, OBJP, SZ, SZ, SZ, P, U64, P]),
+    "_halo2_multiopen_poly_max_len": (SZ, [SZ, SZ, U64]),
+    "_halo2_multiopen_poly_calculation_workspace_size": (U64, [U64, U64]),
+    "_halo2_multiopen_poly_calculation": (RustError, [OBJP, OBJP, SZ, SZ, SZ, OBJP, OBJP, OBJP, P, U64, P]),
+    "_halo2_poly_multiply_add": (RustError, [P, P, P, SZ, P]),
+    "_halo2_poly_elementwise_multiply": (RustError, [P, P, P, U64, P]),
+    "_halo2_poly_fill_one": (RustError, [P, U64, P]),
+    "_halo2_poly_sub_short_inplace": (RustError, [P, P, U64, P]),
+    "_halo2_poly_sub_short_out_of_place": (RustError, [P, P, P, U64, U64, P]),
+    "_halo2_poly_sub_scalar_at_zero": (RustError, [P, P, P]),
+    "_halo2_kate_division_workspace_size": (U64, [U64]),
+    "_halo2_kate_division_device": (RustError, [P, P, P, U64, P, U64, P]),
+    "_halo2_kate_division_device_padded": (RustError, [P, P, P, U64, U64, P, U64, P]),
+    "_halo2_decode_assigned": (RustError, [P, P, P, U64, AssignedLayout, P]),
     "h2amd_random_fr": (RustError, [P, U64, U64, U64, P]),
     "h2amd_g1_mul_generator_workspace_size": (U64, [U64]),
     "h2amd_g1_mul_generator": (RustError, [P, P, U64, P, U64, P]),

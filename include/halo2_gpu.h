@@ -249,11 +249,112 @@ RustError _halo2_quotient_device_columns_device_out(const void* const* fixed_d_p
 
 /* ============================================= omega tables on device */
 
-/* culib.rs:427 ; cuda/src/polynomial/omega.cu */
+/* culib.rs:443 ; cuda/src/polynomial/omega.cu:72-76 */
 uint64_t _halo2_generate_omega_powers_workspace_size(uint32_t log_n);
-/* culib.rs:433-440 ; omega.cu — d_omega_powers[i] = omega^i for i < 2^log_n */
+/* culib.rs:449-456 ; omega.cu:77-97 — d_omega_powers[i] = omega^i for i < 2^log_n */
 RustError _halo2_generate_omega_powers(void* d_omega_powers, const void* d_omega, uint32_t log_n,
     void* scratch, uint64_t scratch_bytes, void* stream);
+
+/* culib.rs:430 ; omega.cu:15-20 */
+uint64_t _halo2_power_of_omega_workspace_size(uint32_t log_n);
+/* culib.rs:432-441 ; omega.cu:29-70 — host omega in, host outputs: omega_lut[0] = 1,
+ * omega_lut[i] = omega^(2^(i-1)) for i = 1..log_n, res = omega^pow. Error when log_n >= 32
+ * or pow >= 2^log_n. Synchronises `stream`. */
+RustError _halo2_power_of_omega(const FFITraitObject* res, const FFITraitObject* omega_lut,
+    const FFITraitObject* omega, uint32_t log_n, uint32_t pow, void* scratch, uint64_t scratch_bytes,
+    void* stream);
+/* culib.rs:458 ; omega.cu:99-105 */
+uint64_t _halo2_generate_omega_lut_workspace_size(uint32_t log_n);
+/* culib.rs:460-467 ; omega.cu:107-151 — host output of 2^10 + 2^(log_n-10) entries:
+ * omega^i for i < 2^10, then omega^(i * 2^10) for i < 2^(log_n - 10). log_n in [10, 42). */
+RustError _halo2_generate_omega_lut(const FFITraitObject* sparse_twiddle, const FFITraitObject* omega,
+    uint32_t log_n, void* scratch, uint64_t scratch_bytes, void* stream);
+/* culib.rs:469-475 ; omega.cu:153-161 */
+uint64_t _halo2_generate_omegadelta_workspace_size(uint32_t log_n, uint32_t omega_start, uint32_t omega_end,
+    uint32_t colunm_num, uint32_t colunm_offset);
+/* culib.rs:477-490 ; omega.cu:163-210, kernel/omega.h:226-260 — device output
+ * out[i - omega_start] = omega^row * delta^col for i in [omega_start, omega_end], with
+ * (col, row) = (mapping[2i], mapping[2i + 1]) a device u32 array; omega and delta are host
+ * 32-byte scalars; delta powers are tabled up to the next power of two >= colunm_num.
+ * log_n in [10, 42). colunm_offset is accepted and unused, as in the reference. */
+RustError _halo2_generate_omegadelta(void* omegadelta, const void* mapping, const void* omega_host,
+    const void* delta_host, uint32_t log_n, uint32_t omega_start, uint32_t omega_end, uint32_t colunm_num,
+    uint32_t colunm_offset, void* scratch, uint64_t scratch_bytes, void* stream);
+
+/* ================================= opening-phase polynomial helpers */
+
+/* culib.rs:231 ; cuda/src/polynomial/polynomial.cu:76-79 */
+uint64_t _halo2_eval_polynomial_workspace_size(uint64_t length);
+/* culib.rs:233-243 ; polynomial.cu:81-108 — p(point) for a device poly of `length`
+ * coefficients and a host point; the 32-byte result stays in `scratch` and its device
+ * address is written to *d_result_out. */
+RustError _halo2_eval_polynomial(const void* d_poly, const FFITraitObject* point, void** d_result_out,
+    uint64_t length, void* scratch, uint64_t scratch_bytes, void* stream);
+/* culib.rs:245-249 ; polynomial.cu:140-152 — largest power-of-two halving of poly_length whose
+ * batch workspace fits free_bytes (0 if none) */
+size_t _halo2_eval_poly_batch_max_len(size_t poly_length, size_t batch_size, uint64_t free_bytes);
+/* culib.rs:251 ; polynomial.cu:154-160 */
+uint64_t _halo2_eval_polynomial_batch_workspace_size(uint64_t poly_length, uint64_t batch_size);
+/* culib.rs:253-264 ; polynomial.cu:162-245 — host polys (coefficients
+ * [poly_offset, poly_offset + poly_length) of each), host points, host results:
+ * eval_result[i] = sum_j poly_i[poly_offset + j] * point_i^j. Synchronises `stream`. */
+RustError _halo2_eval_polynomial_batch(const FFITraitObject* poly_in_many, const FFITraitObject* eval_points,
+    const FFITraitObject* eval_result, size_t poly_offset, size_t poly_length, size_t batch_size, void* scratch,
+    uint64_t scratch_bytes, void* stream);
+/* culib.rs:401-405 ; cuda/src/multiopen.cu:47-57 */
+size_t _halo2_multiopen_poly_max_len(size_t poly_length, size_t batch_size, uint64_t free_bytes);
+/* culib.rs:407-410 ; multiopen.cu:65-71 */
+uint64_t _halo2_multiopen_poly_calculation_workspace_size(uint64_t poly_length, uint64_t batch_size);
+/* culib.rs:412-426 ; multiopen.cu:76-171 — host inputs/outputs over coefficients
+ * [poly_offset, poly_offset + poly_length): poly_acc = sum_i challenge_i * poly_i (the slice
+ * of poly_acc is overwritten) and evaluate_result[i] = slice of poly_i at evaluate_point[i].
+ * Synchronises `stream`. */
+RustError _halo2_multiopen_poly_calculation(const FFITraitObject* poly_in_many, const FFITraitObject* poly_acc,
+    size_t poly_offset, size_t poly_length, size_t batch_size, const FFITraitObject* challenge_point,
+    const FFITraitObject* evaluate_point, const FFITraitObject* evalaute_result, void* scratch,
+    uint64_t scratch_bytes, void* stream);
+/* culib.rs:266-274 ; polynomial.cu:435-455 — d_poly_acc[i] += *d_scalar * d_poly_in[i] */
+RustError _halo2_poly_multiply_add(void* d_poly_acc, const void* d_poly_in, const void* d_scalar,
+    size_t poly_length, void* stream);
+/* culib.rs:276-283 ; polynomial.cu:321-341 — d_out[i] = d_a[i] * d_b[i] (aliasing allowed) */
+RustError _halo2_poly_elementwise_multiply(void* d_out, const void* d_a, const void* d_b, uint64_t length,
+    void* stream);
+/* culib.rs:285-290 ; polynomial.cu:343-361 — d_out[i] = 1 (Montgomery one) */
+RustError _halo2_poly_fill_one(void* d_out, uint64_t length, void* stream);
+/* culib.rs:352-358 ; polynomial.cu:363-385 — d_acc[i] -= d_short[i], i < short_len */
+RustError _halo2_poly_sub_short_inplace(void* d_acc, const void* d_short, uint64_t short_len, void* stream);
+/* culib.rs:360-371 ; polynomial.cu:390-416 — d_out[i] = d_long[i] - d_short[i] for
+ * i < short_len, d_long[i] for short_len <= i < long_len */
+RustError _halo2_poly_sub_short_out_of_place(void* d_out, const void* d_long, const void* d_short,
+    uint64_t short_len, uint64_t long_len, void* stream);
+/* culib.rs:373-380 ; polynomial.cu:418-433 — d_buf[0] -= *d_scalar */
+RustError _halo2_poly_sub_scalar_at_zero(void* d_buf, const void* d_scalar, void* stream);
+
+/* culib.rs:319 ; cuda/src/polynomial/kate_division.cu:27-36 */
+uint64_t _halo2_kate_division_workspace_size(uint64_t n);
+/* culib.rs:321-333 ; kate_division.cu:49-135 — d_q (n - 1 coefficients) =
+ * (a(X) - a(u)) / (X - u) for the n coefficients at d_a and the device scalar d_u;
+ * n < 2 is a no-op. */
+RustError _halo2_kate_division_device(const void* d_a, void* d_q, const void* d_u, uint64_t n, void* scratch,
+    uint64_t scratch_bytes, void* stream);
+/* culib.rs:335-347 ; kate_division.cu:148-250 — same quotient in d_q[0, n - 1), zeros in
+ * [n - 1, out_len); with out_len < n - 1 only the first out_len coefficients are written. */
+RustError _halo2_kate_division_device_padded(const void* d_a, void* d_q, const void* d_u, uint64_t n,
+    uint64_t out_len, void* scratch, uint64_t scratch_bytes, void* stream);
+
+/* culib.rs:20-26 ; cuda/include/kernel/decode_assigned.h:10-14 — byte layout of one
+ * Assigned<F> element: tag byte at offset 0 (0 Zero, 1 Trivial, 2 Rational), numerator
+ * and denominator payloads at the given offsets */
+typedef struct AssignedLayout {
+    uint32_t stride_bytes;
+    uint32_t num_offset;
+    uint32_t denom_offset;
+} AssignedLayout;
+/* culib.rs:292-308 ; cuda/src/polynomial/decode_assigned.cu:28-53 — d_nums[i], d_denoms[i] =
+ * (0, 1) / (x, 1) / (n, d) for Zero / Trivial(x) / Rational(n, d) at d_raw + i * stride.
+ * Payload offsets must be 4-byte aligned and leave 32 bytes inside the stride. */
+RustError _halo2_decode_assigned(void* d_nums, void* d_denoms, const void* d_raw, uint64_t n,
+    AssignedLayout layout, void* stream);
 
 /* ======================== device data generation (no reference counterpart)
  * Helpers used by the benchmark and tests to build large seeded inputs in HBM

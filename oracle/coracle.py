@@ -300,3 +300,20 @@ def quotient_rules(fixed, instance, advice, challenges, constants, y, rules, vp,
                             _p(vp), ctypes.c_size_t(num_vp), ctypes.c_size_t(nq), ctypes.c_int64(rot_scale),
                             ctypes.c_size_t(row_start), ctypes.c_size_t(row_end))
     return out
+
+
+def eval_poly(a, u) -> np.ndarray:
+    """a(u) by Horner (halo2-axiom arithmetic::eval_polynomial)."""
+    a = _c(a).reshape(-1, 4)
+    out = np.zeros(4, dtype=np.uint64)
+    lib().or_eval_poly(_p(out), _p(a) if len(a) else None, ctypes.c_size_t(len(a)), _p(_c(u).reshape(4)))
+    return out
+
+
+def kate_division(a, u) -> np.ndarray:
+    """(a(X) - a(u)) / (X - u) (halo2-axiom arithmetic::kate_division): len(a) - 1 coefficients."""
+    a = _c(a).reshape(-1, 4)
+    q = np.zeros((max(0, len(a) - 1), 4), dtype=np.uint64)
+    if len(a) > 1:
+        lib().or_kate_division(_p(q), _p(a), ctypes.c_size_t(len(a)), _p(_c(u).reshape(4)))
+    return q

@@ -1292,3 +1292,32 @@ void or_quotient_rules(uint64_t* out, const uint64_t* prev, const uint64_t* cons
     }
     free(interm);
 }
+
+/* a(u) = sum a_i u^i by Horner from the top coefficient: halo2-axiom 0.5.3
+ * (git 5e4f0e52) arithmetic::eval_polynomial, the CPU side of the reference's
+ * src/cuda/tests.rs:53-68 and tests/eval_polynomial_horner_device_equivalence.rs. */
+void or_eval_poly(uint64_t* out, const uint64_t* a, size_t n, const uint64_t* u)
+{
+    uint64_t acc[4] = { 0, 0, 0, 0 };
+    for (size_t i = n; i-- > 0;) {
+        f_mul(&FR, acc, acc, u);
+        f_add(&FR, acc, acc, a + 4 * i);
+    }
+    f_copy(out, acc);
+}
+
+/* q = (a(X) - a(u)) / (X - u), n - 1 coefficients: halo2-axiom 0.5.3
+ * arithmetic::kate_division (re-exported at src/cpu/arithmetic.rs:11; compared
+ * with the device path in src/cuda/tests.rs:2830-2896). Walking the coefficients
+ * from the top: lead = a_i - tmp, q_{i-1} = lead, tmp = lead * (-u). */
+void or_kate_division(uint64_t* q, const uint64_t* a, size_t n, const uint64_t* u)
+{
+    uint64_t neg_u[4], tmp[4] = { 0, 0, 0, 0 }, lead[4];
+    const uint64_t zero[4] = { 0, 0, 0, 0 };
+    f_sub(&FR, neg_u, zero, u);
+    for (size_t i = n; i-- > 1;) {
+        f_sub(&FR, lead, a + 4 * i, tmp);
+        f_copy(q + 4 * (i - 1), lead);
+        f_mul(&FR, tmp, lead, neg_u);
+    }
+}

@@ -89,6 +89,10 @@ void or_quotient_rules(uint64_t* out, const uint64_t* prev, const uint64_t* cons
 
 void or_fr_inner_product(uint64_t* out, const uint64_t* a, const uint64_t* b, size_t n);
 
+/* ---- opening-phase polynomial helpers ---- */
+void or_eval_poly(uint64_t* out, const uint64_t* a, size_t n, const uint64_t* u);
+void or_kate_division(uint64_t* q, const uint64_t* a, size_t n, const uint64_t* u);
+
 int or_num_threads(void);
 
 #ifdef __cplusplus

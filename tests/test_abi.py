@@ -65,3 +65,23 @@ def test_host_sizing_entry_points(cdll):
     assert lib._halo2_grand_product_max_len(1 << 20, 1024) == 16
     assert lib._halo2_commit_product_max_len(1 << 20, 0) == 0
     assert lib._halo2_permutation_product_workspace_size(1000) >= 4 * 32 * 1000
+
+
+def test_opening_helper_sizing(cdll):
+    """Host-only sizing of the opening-phase helpers (culib.rs:231-251, 319, 401-410, 430-475)."""
+    from halo2_amd import _lib
+
+    lib = _lib.load(LIB)
+    ws = lib._halo2_eval_polynomial_batch_workspace_size(1 << 20, 8)
+    assert ws >= 2 * 32 << 20  # two staged polynomials
+    assert lib._halo2_eval_poly_batch_max_len(1 << 20, 8, ws + 1) == 1 << 20
+    assert lib._halo2_eval_poly_batch_max_len(1 << 20, 8, ws // 2 + 4096) == 1 << 19
+    assert lib._halo2_eval_poly_batch_max_len(1 << 20, 8, 0) == 0
+    mws = lib._halo2_multiopen_poly_calculation_workspace_size(1 << 20, 8)
+    assert mws >= 3 * 32 << 20  # two staged polynomials and the accumulator
+    assert lib._halo2_multiopen_poly_max_len(1 << 20, 8, mws + 1) == 1 << 20
+    assert lib._halo2_kate_division_workspace_size(0) == 0 and lib._halo2_kate_division_workspace_size(1) == 0
+    assert 0 < lib._halo2_kate_division_workspace_size(1 << 20) < 32 << 20  # no full-length state copy
+    assert lib._halo2_eval_polynomial_workspace_size(1 << 24) < 1 << 20
+    assert lib._halo2_generate_omega_lut_workspace_size(20) >= 32 * ((1 << 10) + (1 << 10))
+    assert lib._halo2_power_of_omega_workspace_size(31) >= 32 * 32

@@ -243,3 +243,56 @@ def test_quotient_lookups_vs_bigint(co, po):
         v = (v * y + (a[i] - s[i]) * (a[i] - a[rp]) * la[i]) % R
         exp.append(v)
     assert got == exp
+
+
+def test_eval_poly_closed_forms(co, po):
+    """Horner evaluation (halo2-axiom arithmetic::eval_polynomial): the geometric series
+    sum u^i = (u^n - 1)/(u - 1), u = 0 gives a_0, u = 1 gives sum a_i, and a product of
+    linear factors vanishes at each root while matching prod (z - r_j) elsewhere."""
+    R = po.R
+    rng = random.Random(11)
+    for n in (1, 2, 7, 64, 257):
+        u = rng.randrange(2, R)
+        ones = po.fr_array([1] * n)
+        got = po.fr_values(co.eval_poly(ones, po.fr_array([u])[0]).reshape(1, 4))[0]
+        assert got == (pow(u, n, R) - 1) * pow(u - 1, -1, R) % R
+        vals = [rng.randrange(R) for _ in range(n)]
+        a = po.fr_array(vals)
+        assert po.fr_values(co.eval_poly(a, po.fr_array([0])[0]).reshape(1, 4))[0] == vals[0]
+        assert po.fr_values(co.eval_poly(a, po.fr_array([1])[0]).reshape(1, 4))[0] == sum(vals) % R
+    assert po.fr_values(co.eval_poly(np.zeros((0, 4), dtype=np.uint64), po.fr_array([5])[0]).reshape(1, 4))[0] == 0
+    roots = [rng.randrange(R) for _ in range(9)]
+    coeffs = [1]
+    for r in roots:  # multiply by (X - r)
+        coeffs = [((coeffs[i - 1] if i > 0 else 0) - r * (coeffs[i] if i < len(coeffs) else 0)) % R
+                  for i in range(len(coeffs) + 1)]
+    a = po.fr_array(coeffs)
+    for r in roots:
+        assert po.fr_values(co.eval_poly(a, po.fr_array([r])[0]).reshape(1, 4))[0] == 0
+    z = rng.randrange(R)
+    expect = 1
+    for r in roots:
+        expect = expect * (z - r) % R
+    assert po.fr_values(co.eval_poly(a, po.fr_array([z])[0]).reshape(1, 4))[0] == expect
+
+
+def test_kate_division_identity(co, po):
+    """kate_division (halo2-axiom arithmetic::kate_division): q (X - u) + a(u) = a(X)
+    coefficient by coefficient, and X^n - u^n divides to sum u^(n-1-i) X^i."""
+    R = po.R
+    rng = random.Random(12)
+    for n in (2, 3, 17, 200):
+        for u in (0, 1, rng.randrange(R)):
+            vals = [rng.randrange(R) for _ in range(n)]
+            q = po.fr_values(co.kate_division(po.fr_array(vals), po.fr_array([u])[0]))
+            assert len(q) == n - 1
+            au = sum(v * pow(u, i, R) for i, v in enumerate(vals)) % R
+            assert (au - u * q[0]) % R == vals[0]
+            for i in range(1, n - 1):
+                assert (q[i - 1] - u * q[i]) % R == vals[i]
+            assert q[n - 2] == vals[n - 1]
+        u = rng.randrange(R)
+        xn = [(-pow(u, n, R)) % R] + [0] * (n - 1) + [1]
+        q = po.fr_values(co.kate_division(po.fr_array(xn), po.fr_array([u])[0]))
+        assert q == [pow(u, n - 1 - i, R) for i in range(n)]
+    assert co.kate_division(po.fr_array([5]), po.fr_array([3])[0]).shape == (0, 4)
