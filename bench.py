@@ -118,6 +118,37 @@ def row_measurements(lib, dv, F, msm, torch, stream) -> dict:
     rec("coeff_to_extended_2^20_to_2^22", 1 << k,
         _timed(lambda: dom.coeff_to_extended_many_device([poly]), 5, torch), 160)
     del cols, poly
+    # opening phase: evaluation at a point and division by (X - u) of a 2^22-coefficient polynomial
+    import numpy as np
+
+    from halo2_amd.lookup import permute_expression_pair_device
+
+    k = 22
+    n = 1 << k
+    poly = msm.random_fr(n, seed=360)
+    u = dv.to_device(F.mont_limbs(0x1234567).reshape(1, 4))
+    ews = int(lib._halo2_eval_polynomial_workspace_size(n))
+    escr = dv.empty_bytes(ews)
+    res = ctypes.c_void_p()
+    pt = np.ascontiguousarray(F.mont_limbs(0x1234567).reshape(1, 4))
+    rec("eval_polynomial_2^22", n, _timed(lambda: _lib.check(lib._halo2_eval_polynomial(
+        dv.ptr(poly), ctypes.byref(_lib.obj(pt.ctypes.data)), ctypes.byref(res), n, dv.ptr(escr), ews, stream)),
+        5, torch), 32)
+    q = dv.empty_u64(4 * n)
+    kws = int(lib._halo2_kate_division_workspace_size(n))
+    kscr = dv.empty_bytes(kws)
+    rec("kate_division_2^22", n, _timed(lambda: _lib.check(lib._halo2_kate_division_device(
+        dv.ptr(poly), dv.ptr(q), dv.ptr(u), n, dv.ptr(kscr), kws, stream)), 5, torch), 64)
+    # lookup permutation pair: mixed duplicates, usable rows 2^22 - 6
+    alphabet = msm.random_fr(n // 3, seed=361)
+    g = torch.Generator(device="cpu").manual_seed(362)
+    inp = alphabet.view(-1, 4)[torch.randint(0, n // 3, (n,), generator=g).to(alphabet.device)].reshape(-1)
+    tab = torch.cat([alphabet, alphabet.view(-1, 4)[torch.randint(
+        0, n // 3, (n - n // 3,), generator=g).to(alphabet.device)].reshape(-1)])
+    pin, ptab = dv.empty_u64(4 * n), dv.empty_u64(4 * n)
+    rec("permute_expression_pair_2^22", n, _timed(
+        lambda: permute_expression_pair_device(inp, tab, pin, ptab, n, n - 6), 3, torch), 128)
+    del poly, q, inp, tab, pin, ptab, alphabet
     # MSM at 2^16 (the small-primitive size)
     m = 1 << 16
     pts = msm.g1_mul_generator(msm.random_fr(m, seed=350), m)

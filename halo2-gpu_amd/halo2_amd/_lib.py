@@ -84,6 +84,8 @@ SIGNATURES = {
     "_halo2_commit_product_workspace_size": (U64, [U64]),
     "_halo2_commit_product": (RustError, [OBJP, OBJP, OBJP, OBJP, OBJP, P, P, SZ, SZ, P, U64, P]),
     "_halo2_commit_product_device_inputs": (RustError, [OBJP, OBJP, OBJP, OBJP, OBJP, P, P, SZ, SZ, P]),
+    "_halo2_permute_expression_pair_workspace_size": (U64, [U64, U64]),
+    "_halo2_permute_expression_pair": (RustError, [P, P, P, P, U64, U64, P, U64, P]),
     "_halo2_permutation_product_max_len": (SZ, [SZ, U64]),
     "_halo2_permutation_product_workspace_size": (U64, [U64]),
     "_halo2_permutation_product": (RustError, [OBJP, OBJP, OBJP, P, P, P, P, P, SZ, SZ, SZ, P, U64, P]),

@@ -1,0 +1,191 @@
+"""SHPLONK multi-opening on the device, mirroring the GPU arm of the reference's
+ProverSHPLONK::create_proof (src/poly/kzg/multiopen/shplonk/prover.rs:97-441) and
+construct_intermediate_sets (src/poly/kzg/multiopen/shplonk.rs:51-160).
+
+The transcript is not part of this path: the challenges y, v, u that the
+reference squeezes from it are arguments here, and the two commitments it
+writes (h, then h_final) are returned. Device polynomials are int64 torch
+tensors of 4*n words (coefficient form); points and challenges are Python ints.
+The small host folds (Lagrange interpolation over at most a few points, the
+vanishing products) run on Python integers, as the reference runs them on the
+CPU.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import device as dv
+from . import polyops
+from .fields import R_MOD, mont_limbs, value_of
+
+
+@dataclass
+class ProverQuery:
+    """src/poly/query.rs ProverQuery: open `poly` (device coefficients) at `point`;
+    `commitment` identifies the committed polynomial (queries on one polynomial share it)."""
+
+    point: int
+    poly: torch.Tensor
+    commitment: object
+
+
+@dataclass
+class _Commitment:
+    key: object
+    poly: torch.Tensor
+    evals: list[int]
+    low_degree_equivalent: list[int] | None = None
+
+
+@dataclass
+class RotationSet:
+    points: list[int]
+    commitments: list[_Commitment]
+
+
+def powers(x: int, count: int) -> list[int]:
+    out, cur = [], 1
+    for _ in range(count):
+        out.append(cur)
+        cur = cur * x % R_MOD
+    return out
+
+
+def evaluate_vanishing_polynomial(roots, z: int) -> int:
+    """prod (z - root) (halo2-axiom arithmetic::evaluate_vanishing_polynomial)."""
+    acc = 1
+    for r in roots:
+        acc = acc * (z - r) % R_MOD
+    return acc
+
+
+def lagrange_interpolate(points: list[int], evals: list[int]) -> list[int]:
+    """Coefficients of the unique polynomial of degree < len(points) through (points, evals)
+    (halo2-axiom arithmetic::lagrange_interpolate)."""
+    m = len(points)
+    out = [0] * m
+    for j in range(m):
+        basis = [1]
+        denom = 1
+        for k in range(m):
+            if k == j:
+                continue
+            basis = [((basis[i - 1] if i > 0 else 0) - points[k] * (basis[i] if i < len(basis) else 0)) % R_MOD
+                     for i in range(len(basis) + 1)]
+            denom = denom * (points[j] - points[k]) % R_MOD
+        scale = evals[j] * pow(denom, -1, R_MOD) % R_MOD
+        for i, c in enumerate(basis):
+            out[i] = (out[i] + c * scale) % R_MOD
+    return out
+
+
+def eval_host(coeffs: list[int], x: int) -> int:
+    acc = 0
+    for c in reversed(coeffs):
+        acc = (acc * x + c) % R_MOD
+    return acc
+
+
+def construct_intermediate_sets(queries: list[ProverQuery]) -> tuple[list[RotationSet], list[int]]:
+    """shplonk.rs:51-160: evaluate every query once (batched on the device), group the
+    points per commitment, then the commitments per point set. Commitment keys must be
+    hashable. Points keep their first appearance order (the reference iterates hash
+    sets; the results do not depend on the order)."""
+    evals = polyops.batch_eval_polynomial_d2h([q.poly for q in queries], [q.point for q in queries])
+    eval_of = {(q.commitment, q.point % R_MOD): value_of(e) for q, e in zip(queries, evals)}
+    super_points: list[int] = []
+    per_commitment: dict = {}  # key -> (poly, points), first-appearance order
+    for q in queries:
+        p = q.point % R_MOD
+        if p not in super_points:
+            super_points.append(p)
+        _, pts = per_commitment.setdefault(q.commitment, (q.poly, []))
+        if p not in pts:
+            pts.append(p)
+    sets: list[RotationSet] = []
+    for key, (poly, pts) in per_commitment.items():
+        for rs in sets:
+            if set(rs.points) == set(pts):
+                rs.commitments.append(_Commitment(key, poly, [eval_of[(key, p)] for p in rs.points]))
+                break
+        else:
+            sets.append(RotationSet(list(pts), [_Commitment(key, poly, [eval_of[(key, p)] for p in pts])]))
+    return sets, super_points
+
+
+def _dev_scalar(x: int) -> torch.Tensor:
+    return dv.to_device(mont_limbs(x % R_MOD).reshape(1, 4))
+
+
+def _dev_table(vals: list[int]) -> torch.Tensor:
+    return dv.to_device(np.stack([mont_limbs(v % R_MOD) for v in vals]))
+
+
+def _zeros(n: int) -> torch.Tensor:
+    return torch.zeros(4 * n, dtype=torch.int64, device=dv.device())
+
+
+def create_proof_device(params, queries: list[ProverQuery], y: int, v: int, u: int) -> dict:
+    """prover.rs:97-441 with the transcript's challenges given: returns the two commitments
+    ("h", "h_final", 96-byte projective points as (12,) u64) and the device polynomials
+    h_x (n coefficients) and h_x_final (n - 1 coefficients)."""
+    n = params.n
+    rotation_sets, super_points = construct_intermediate_sets(queries)
+    for rs in rotation_sets:
+        for c in rs.commitments:
+            c.low_degree_equivalent = lagrange_interpolate(rs.points, c.evals)
+    max_c = max([len(rs.commitments) for rs in rotation_sets] + [1])
+    d_y_powers = _dev_table(powers(y, max_c))
+
+    quotient_polys, p_xs = [], []
+    for rs in rotation_sets:  # quotient_contribution, prover.rs:183-268
+        d_p_x = _zeros(n)
+        for i, c in enumerate(rs.commitments):
+            d_in = c.poly if c.poly.numel() == 4 * n else torch.cat([c.poly, _zeros(n - c.poly.numel() // 4)])
+            polyops.poly_multiply_add_device_at_lut_offset(d_p_x, d_in, d_y_powers, i)
+        r_x = [0] * max(len(c.low_degree_equivalent) for c in rs.commitments)
+        for c, py in zip(rs.commitments, powers(y, len(rs.commitments))):
+            for i, coef in enumerate(c.low_degree_equivalent):
+                r_x[i] = (r_x[i] + coef * py) % R_MOD
+        d_n_x = dv.empty_u64(4 * n)
+        polyops.poly_sub_short_out_of_place_device(d_n_x, d_p_x, _dev_table(r_x))
+        d_q = d_n_x
+        for i, point in enumerate(rs.points):
+            d_root = _dev_scalar(point)
+            if i + 1 == len(rs.points):
+                d_q = polyops.kate_division_device_padded_with_d_root(d_q, d_root, n)
+            else:
+                d_q = polyops.kate_division_device_with_d_root(d_q, d_root)
+        quotient_polys.append(d_q)
+        p_xs.append(d_p_x)
+
+    d_v_powers = _dev_table(powers(v, max(1, len(quotient_polys))))
+    d_h_x = _zeros(n)
+    for j, q in enumerate(quotient_polys):
+        polyops.poly_multiply_add_device_at_lut_offset(d_h_x, q, d_v_powers, j)
+    h = params.commit_device(d_h_x)
+
+    lin, z_diffs = [], []
+    for rs, d_l_x in zip(rotation_sets, p_xs):  # linearisation_contribution, prover.rs:331-386
+        diffs = [p for p in super_points if p not in rs.points]
+        z_i = evaluate_vanishing_polynomial(diffs, u)
+        r_i = 0
+        for c, py in zip(rs.commitments, powers(y, len(rs.commitments))):
+            r_i = (r_i + eval_host(c.low_degree_equivalent, u) * py) % R_MOD
+        polyops.poly_sub_scalar_at_zero_device(d_l_x, _dev_scalar(r_i))
+        polyops.poly_scale_device_with_d_s_minus_one(d_l_x, _dev_scalar(z_i - 1))
+        lin.append(d_l_x)
+        z_diffs.append(z_i)
+    d_l_x = _zeros(n)
+    for j, poly in enumerate(lin):
+        polyops.poly_multiply_add_device_at_lut_offset(d_l_x, poly, d_v_powers, j)
+    zt_eval = evaluate_vanishing_polynomial(super_points, u)
+    polyops.poly_multiply_add_device(d_l_x, d_h_x, -zt_eval % R_MOD)
+    d_h_final = polyops.kate_division_device(d_l_x, u)
+    z0_inv = pow(z_diffs[0], -1, R_MOD)
+    polyops.poly_scale_device_with_d_s_minus_one(d_h_final, _dev_scalar(z0_inv - 1))
+    h_final = params.commit_device(d_h_final)
+    return {"h": h, "h_final": h_final, "h_x": d_h_x, "h_x_final": d_h_final, "l_x": d_l_x}

@@ -143,6 +143,15 @@ RustError _halo2_distribute_powers_zeta(void* d_a, const void* d_coset_powers, u
  * field_type 0 = Fr, 1 = Fq */
 RustError _halo2_batch_invert(void* d_data, uint32_t field_type, size_t length, void* stream);
 
+/* culib.rs:127 ; cuda/src/commitment/permute_expression_pair.cu:202-227 */
+uint64_t _halo2_permute_expression_pair_workspace_size(uint64_t n, uint64_t usable_rows);
+/* culib.rs:129-141 ; permute_expression_pair.cu:229-350 — device in/out, n rows each: the
+ * lookup permutation pair of src/plonk/lookup/prover.rs:788-882 (sorted input by canonical
+ * value; run heads take their own value; leftover table values ascending fill the repeated
+ * rows from the last one backwards; rows [usable_rows, n) zero). Error if usable_rows > n. */
+RustError _halo2_permute_expression_pair(const void* d_compressed_input, const void* d_compressed_table,
+    void* d_permuted_input, void* d_permuted_table, uint64_t n, uint64_t usable_rows, void* scratch,
+    uint64_t scratch_bytes, void* stream);
 /* culib.rs:82 ; cuda/src/commitment/commit_product.cu:33-43 */
 size_t _halo2_commit_product_max_len(size_t length, uint64_t free_bytes);
 /* culib.rs:83 ; commit_product.cu:45-48 */

@@ -317,3 +317,16 @@ def kate_division(a, u) -> np.ndarray:
     if len(a) > 1:
         lib().or_kate_division(_p(q), _p(a), ctypes.c_size_t(len(a)), _p(_c(u).reshape(4)))
     return q
+
+
+def permute_expression_pair(inp, table, n: int, usable_rows: int):
+    """(permuted_input, permuted_table), n rows each (src/plonk/lookup/prover.rs:788-882);
+    raises ValueError where the reference panics (an input value missing from the table)."""
+    a, t = _c(inp).reshape(-1, 4), _c(table).reshape(-1, 4)
+    pin = np.zeros((n, 4), dtype=np.uint64)
+    ptab = np.zeros((n, 4), dtype=np.uint64)
+    err = lib().or_permute_expression_pair(_p(pin), _p(ptab), _p(a), _p(t), ctypes.c_size_t(n),
+                                           ctypes.c_size_t(usable_rows))
+    if err:
+        raise ValueError("input value missing from the table")
+    return pin, ptab

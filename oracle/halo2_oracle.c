@@ -1321,3 +1321,89 @@ void or_kate_division(uint64_t* q, const uint64_t* a, size_t n, const uint64_t* 
         f_mul(&FR, tmp, lead, neg_u);
     }
 }
+
+/* ---- lookup permutation pair (src/plonk/lookup/prover.rs:788-882, permute_expression_pair_seq) ---- */
+
+typedef struct {
+    uint64_t c[4]; /* canonical value */
+    uint64_t m[4]; /* Montgomery form as stored */
+} perm_key;
+
+static int perm_cmp(const void* pa, const void* pb)
+{
+    const perm_key* a = (const perm_key*)pa;
+    const perm_key* b = (const perm_key*)pb;
+    for (int i = 3; i >= 0; --i)
+        if (a->c[i] != b->c[i])
+            return a->c[i] < b->c[i] ? -1 : 1;
+    return 0;
+}
+
+/* permuted_input = the first U inputs sorted by canonical value; at the first row
+ * of each run of equal inputs permuted_table takes that value and one table
+ * occurrence of it is consumed; the leftover table values, ascending, fill the
+ * repeated rows from the last one backwards; rows [U, n) are zero. Returns 1
+ * (outputs unspecified) when an input value is missing from the table, where
+ * the reference panics. */
+int or_permute_expression_pair(uint64_t* pin, uint64_t* ptab, const uint64_t* input, const uint64_t* table,
+                               size_t n, size_t U)
+{
+    memset(pin, 0, 32 * n);
+    memset(ptab, 0, 32 * n);
+    if (U == 0)
+        return 0;
+    perm_key* in = (perm_key*)malloc(U * sizeof(perm_key));
+    perm_key* tab = (perm_key*)malloc(U * sizeof(perm_key));
+    uint32_t* cnt = (uint32_t*)malloc(U * sizeof(uint32_t));
+    size_t* rep = (size_t*)malloc(U * sizeof(size_t));
+    for (size_t i = 0; i < U; ++i) {
+        f_copy(in[i].m, input + 4 * i);
+        f_from_mont(&FR, in[i].c, input + 4 * i);
+        f_copy(tab[i].m, table + 4 * i);
+        f_from_mont(&FR, tab[i].c, table + 4 * i);
+    }
+    qsort(in, U, sizeof(perm_key), perm_cmp);
+    qsort(tab, U, sizeof(perm_key), perm_cmp);
+    /* distinct table values (ascending) with their counts, compacted to the front */
+    size_t nd = 0;
+    for (size_t j = 0; j < U; ++j) {
+        if (nd > 0 && perm_cmp(&tab[nd - 1], &tab[j]) == 0) {
+            ++cnt[nd - 1];
+        } else {
+            tab[nd] = tab[j];
+            cnt[nd] = 1;
+            ++nd;
+        }
+    }
+    int err = 0;
+    size_t nrep = 0;
+    for (size_t row = 0; row < U; ++row) {
+        f_copy(pin + 4 * row, in[row].m);
+        if (row == 0 || perm_cmp(&in[row], &in[row - 1]) != 0) {
+            f_copy(ptab + 4 * row, in[row].m);
+            perm_key* hit = (perm_key*)bsearch(&in[row], tab, nd, sizeof(perm_key), perm_cmp);
+            if (!hit) {
+                err = 1;
+                break;
+            }
+            --cnt[hit - tab];
+        } else {
+            rep[nrep++] = row;
+        }
+    }
+    for (size_t d = 0; d < nd && !err; ++d)
+        for (uint32_t c = 0; c < cnt[d]; ++c) {
+            if (nrep == 0) {
+                err = 1;
+                break;
+            }
+            f_copy(ptab + 4 * rep[--nrep], tab[d].m);
+        }
+    if (nrep)
+        err = 1;
+    free(in);
+    free(tab);
+    free(cnt);
+    free(rep);
+    return err;
+}

@@ -93,6 +93,10 @@ void or_fr_inner_product(uint64_t* out, const uint64_t* a, const uint64_t* b, si
 void or_eval_poly(uint64_t* out, const uint64_t* a, size_t n, const uint64_t* u);
 void or_kate_division(uint64_t* q, const uint64_t* a, size_t n, const uint64_t* u);
 
+/* ---- lookup permutation pair; returns 1 when an input value is missing from the table ---- */
+int or_permute_expression_pair(uint64_t* pin, uint64_t* ptab, const uint64_t* input, const uint64_t* table,
+                               size_t n, size_t U);
+
 int or_num_threads(void);
 
 #ifdef __cplusplus

@@ -296,3 +296,63 @@ def test_kate_division_identity(co, po):
         q = po.fr_values(co.kate_division(po.fr_array(xn), po.fr_array([u])[0]))
         assert q == [pow(u, n - 1 - i, R) for i in range(n)]
     assert co.kate_division(po.fr_array([5]), po.fr_array([3])[0]).shape == (0, 4)
+
+
+def _perm_case(rng, R, U, pattern):
+    if U == 0:
+        return [], []
+    alpha = {"distinct": U, "mixed": max(1, U // 3), "same": 1, "small": min(U, 7)}[pattern]
+    if pattern == "small":  # small canonical values: keys differ only in the low word
+        alphabet = list(range(alpha))
+    else:
+        alphabet = [rng.randrange(R) for _ in range(alpha)]
+    table = alphabet + [rng.choice(alphabet) for _ in range(U - alpha)]
+    rng.shuffle(table)
+    if pattern == "distinct":
+        inp = alphabet[:]
+        rng.shuffle(inp)
+    else:
+        inp = [rng.choice(alphabet) for _ in range(U)]
+    return inp, table
+
+
+def test_permute_expression_pair_seq(co, po):
+    """The oracle against (1) the lookup argument's own constraints on the result:
+    permuted input sorted and a permutation of the input, permuted table a permutation
+    of the table, row 0 equal, and every row either equal or repeating the row above;
+    and (2) a big-integer restatement of the reference's sequential algorithm (sorted,
+    run heads consume one table occurrence, leftovers ascending from the last repeated
+    row) on the reference test's patterns (src/cuda/tests.rs:2340-2548)."""
+    R = po.R
+    rng = random.Random(21)
+    for U, n, pattern in ((1, 4, "distinct"), (13, 16, "distinct"), (100, 128, "mixed"), (50, 64, "same"),
+                          (120, 128, "small"), (0, 8, "mixed")):
+        inp, table = _perm_case(rng, R, U, pattern)
+        pad_in = inp + [rng.randrange(R) for _ in range(n - U)]
+        pad_tab = table + [rng.randrange(R) for _ in range(n - U)]
+        pin, ptab = co.permute_expression_pair(po.fr_array(pad_in), po.fr_array(pad_tab), n, U)
+        A, S = po.fr_values(pin), po.fr_values(ptab)
+        assert A[U:] == [0] * (n - U) and S[U:] == [0] * (n - U)
+        assert A[:U] == sorted(inp) and sorted(S[:U]) == sorted(table)
+        for i in range(U):
+            assert A[i] == S[i] or (i > 0 and A[i] == A[i - 1])
+        if U:
+            assert A[0] == S[0]
+        # sequential restatement in big integers
+        srt = sorted(inp)
+        left = {}
+        for v in table:
+            left[v] = left.get(v, 0) + 1
+        exp, rep = [0] * U, []
+        for r, v in enumerate(srt):
+            if r == 0 or v != srt[r - 1]:
+                exp[r] = v
+                left[v] -= 1
+            else:
+                rep.append(r)
+        for v in sorted(left):
+            for _ in range(left[v]):
+                exp[rep.pop()] = v
+        assert S[:U] == exp
+    with pytest.raises(ValueError):
+        co.permute_expression_pair(po.fr_array([1, 2]), po.fr_array([1, 1]), 2, 2)
