@@ -307,34 +307,107 @@ __global__ void __launch_bounds__(R1_THREADS) r1_count_kernel(const uint32_t* __
         h1[(uint64_t)b * sp.ntiles + blockIdx.x] = hist[b];
 }
 
-// Round 1, pass 2: the same tile places its entries at the scanned offsets,
-// one 8-byte store per entry.
+// Round 1, pass 2: the same tile places its entries at the scanned offsets.
+// Window by window the tile's entries are first ordered by coarse bin in LDS
+// (LDS ranks, a scan of the 2^CB bin counts), then written out in that order,
+// so consecutive threads store consecutive addresses of one bin's run instead
+// of isolated 8-byte words. A (window, bin) pair is written in one window
+// iteration only, so the per-tile cursors need no update.
+uint32_t r1_scatter_lds(uint32_t NC, uint32_t CB)
+{
+    return 4 * ((NC + 1) & ~1u) + 4 * (((1u << CB) + 2) & ~1u) + 10 * R1_TILE; // < 56 KiB for c <= 20
+}
+
 __global__ void __launch_bounds__(R1_THREADS) r1_scatter_kernel(const uint32_t* __restrict__ canon, SortParams sp,
     const uint32_t* __restrict__ h1off, uint64_t* __restrict__ kv1)
 {
-    extern __shared__ uint32_t cursor[];
+    extern __shared__ uint32_t sm[];
+    const uint32_t BW = 1u << sp.CB;
+    uint32_t* cursor = sm; // NC: this tile's start in every (window, bin) run
+    uint32_t* base = cursor + ((sp.NC + 1) & ~1u); // BW + 1: per-window bin counts, then their exclusive scan
+    uint64_t* stage = (uint64_t*)(base + ((BW + 2) & ~1u)); // R1_TILE entries, bin-ordered
+    uint16_t* sbin = (uint16_t*)(stage + R1_TILE); // the bin of each staged entry
+    __shared__ uint32_t wsum[R1_THREADS / 64];
     for (uint32_t b = threadIdx.x; b < sp.NC; b += R1_THREADS)
         cursor[b] = h1off[(uint64_t)b * sp.ntiles + blockIdx.x];
-    __syncthreads();
     const uint32_t fmask = (1u << sp.FB) - 1u;
     const uint64_t p0 = (uint64_t)blockIdx.x * R1_TILE;
+    uint32_t s[R1_PER][8];
+    uint32_t carry[R1_PER];
+#pragma unroll
     for (uint32_t k = 0; k < R1_PER; ++k) {
         const uint64_t i = p0 + (uint64_t)k * R1_THREADS + threadIdx.x;
-        if (i >= sp.n)
-            break;
-        uint32_t s[8];
+        carry[k] = 0;
 #pragma unroll
         for (int q = 0; q < 8; ++q)
-            s[q] = canon[8 * i + q];
-        uint32_t carry = 0;
-        for (uint32_t w = 0; w < sp.W; ++w) {
-            const int32_t d = next_digit(s, w, sp.c, carry);
+            s[k][q] = i < sp.n ? canon[8 * i + q] : 0u; // a zero scalar has only zero digits
+    }
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t w = 0; w < sp.W; ++w) {
+        for (uint32_t b = threadIdx.x; b <= BW; b += R1_THREADS)
+            base[b] = 0;
+        __syncthreads();
+        uint32_t bin[R1_PER], rank[R1_PER];
+        uint64_t ent[R1_PER];
+#pragma unroll
+        for (uint32_t k = 0; k < R1_PER; ++k) {
+            const uint64_t i = p0 + (uint64_t)k * R1_THREADS + threadIdx.x;
+            const int32_t d = next_digit(s[k], w, sp.c, carry[k]);
+            bin[k] = BW; // no entry
             if (d != 0) {
                 const uint32_t bi = (uint32_t)(d < 0 ? -d : d) - 1u;
-                const uint32_t pos = atomicAdd(&cursor[(w << sp.CB) + (bi >> sp.FB)], 1u);
-                kv1[pos] = ((uint64_t)(bi & fmask) << 32) | (uint32_t)i | (d < 0 ? 0x80000000u : 0u);
+                bin[k] = bi >> sp.FB;
+                ent[k] = ((uint64_t)(bi & fmask) << 32) | (uint32_t)i | (d < 0 ? 0x80000000u : 0u);
+                rank[k] = atomicAdd(&base[bin[k]], 1u);
             }
         }
+        __syncthreads();
+        // exclusive scan of base[0, BW) in place, base[BW] = total: each thread
+        // owns a contiguous run of ceil(BW / R1_THREADS) counters
+        const uint32_t per = (BW + R1_THREADS - 1) / R1_THREADS;
+        const uint32_t b0 = threadIdx.x * per;
+        uint32_t run = 0;
+        for (uint32_t q = 0; q < per; ++q)
+            if (b0 + q < BW)
+                run += base[b0 + q];
+        uint32_t incl = run;
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            const uint32_t t = __shfl_up(incl, d, 64);
+            if (lane >= d)
+                incl += t;
+        }
+        if (lane == 63)
+            wsum[wave] = incl;
+        __syncthreads();
+        uint32_t woff = 0;
+        for (uint32_t q = 0; q < wave; ++q)
+            woff += wsum[q];
+        uint32_t acc = woff + incl - run;
+        for (uint32_t q = 0; q < per; ++q) {
+            if (b0 + q < BW) {
+                const uint32_t c = base[b0 + q];
+                base[b0 + q] = acc;
+                acc += c;
+            }
+        }
+        if (threadIdx.x == R1_THREADS - 1)
+            base[BW] = acc;
+        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < R1_PER; ++k) {
+            if (bin[k] < BW) {
+                const uint32_t idx = base[bin[k]] + rank[k];
+                stage[idx] = ent[k];
+                sbin[idx] = (uint16_t)bin[k];
+            }
+        }
+        __syncthreads();
+        const uint32_t total = base[BW];
+        for (uint32_t idx = threadIdx.x; idx < total; idx += R1_THREADS) {
+            const uint32_t b = sbin[idx];
+            kv1[cursor[(w << sp.CB) + b] + (idx - base[b])] = stage[idx];
+        }
+        __syncthreads();
     }
 }
 
@@ -744,7 +817,7 @@ RustError msm_device(const MsmPlan& p, const MsmBuffers& b, const Fr* d_scalars,
     RustError e = scan_u32_exclusive(b.h1, b.h1off, nh1, b.scan_scratch, stream);
     if (e.code)
         return e;
-    r1_scatter_kernel<<<p.ntiles, R1_THREADS, 4 * p.NC, stream>>>(b.canon, sp, b.h1off, b.kv1);
+    r1_scatter_kernel<<<p.ntiles, R1_THREADS, r1_scatter_lds(p.NC, p.CB), stream>>>(b.canon, sp, b.h1off, b.kv1);
     H2_LAUNCH_CHECK();
     bin_info_kernel<<<1, 1024, 0, stream>>>(b.h1off, sp, b.bin_start, b.chunk_base, b.off0 + p.nb);
     H2_LAUNCH_CHECK();

@@ -128,22 +128,64 @@ def omega_powers(base: int, log_n: int) -> torch.Tensor:
 
 class ParamsKZG:
     """src/poly/kzg/commitment.rs:27-582: KZG parameters with the G1 SRS kept resident
-    in HBM (`g_device`). Built from a toxic-waste tau (`setup`, :146-230): g[i] = tau^i G.
-    The Lagrange basis and G2 elements are not part of this hot path."""
+    in HBM. Built from a toxic-waste tau (`setup`, :118-197): g[i] = tau^i G and the
+    Lagrange basis g_lagrange[i] = L_i(tau) G = G * ((tau^n - 1)/n) * w^i / (tau - w^i),
+    both generated on the device. The lazy device mirrors of the reference
+    (`g_device`, `g_lagrange_device`, :246-304) are the resident buffers themselves.
+    G2 elements belong to the verifier and are not part of this path."""
 
-    def __init__(self, k: int, tau: int):
+    def __init__(self, k: int, tau: int, omega: int | None = None):
+        from .fields import omega as root
+
         self.k = k
         self.n = 1 << k
         self.tau = tau % R_MOD
+        self.omega = root(k) if omega is None else omega % R_MOD
         self.g_device = g1_mul_generator(omega_powers(self.tau, k), self.n)
+        self._g_lagrange_device = None
+
+    @property
+    def g_lagrange_device(self) -> torch.Tensor:
+        """commitment.rs:157-183 (scalars) and :223-241 (device mirror, built on first use)."""
+        if self._g_lagrange_device is None:
+            from . import polyops
+
+            n = self.n
+            roots = omega_powers(self.omega, self.k)
+            diff = dv.empty_u64(4 * n)
+            polyops.poly_fill_one_device(diff)  # diff = tau - w^i
+            polyops.poly_scale_device_with_d_s_minus_one(diff, _scalar_dev(self.tau - 1))
+            polyops.poly_sub_short_out_of_place_device(diff, diff, roots)
+            _lib.check(_lib.load()._halo2_batch_invert(dv.ptr(diff), 0, n, dv.stream()), "_halo2_batch_invert")
+            polyops.poly_elementwise_multiply_device(diff, diff, roots)
+            mult = (pow(self.tau, n, R_MOD) - 1) * pow(n, -1, R_MOD) % R_MOD
+            polyops.poly_scale_device_with_d_s_minus_one(diff, _scalar_dev(mult - 1))
+            self._g_lagrange_device = g1_mul_generator(diff, n)
+        return self._g_lagrange_device
 
     def commit_device(self, d_poly: torch.Tensor) -> np.ndarray:
-        """commitment.rs:546-577: MSM of device coefficients against the resident SRS."""
+        """commitment.rs:560-577: MSM of device coefficients against the resident SRS."""
         n = d_poly.numel() // 4
         assert n <= self.n
         return multiexp_device_scalars_device_bases(d_poly, self.g_device[: 8 * n], n)
 
     def commit(self, h_poly: np.ndarray) -> np.ndarray:
-        """commitment.rs:456-470: host coefficients, resident SRS."""
+        """commitment.rs:531-541: host coefficients, resident SRS."""
         h = np.ascontiguousarray(h_poly, dtype=np.uint64).reshape(-1, 4)
         return multiexp_device_bases(h, self.g_device[: 8 * len(h)])
+
+    def commit_lagrange_device(self, d_values: torch.Tensor) -> np.ndarray:
+        """commitment.rs:467-485: MSM of device Lagrange values against the resident
+        Lagrange basis."""
+        n = d_values.numel() // 4
+        assert n <= self.n
+        return multiexp_device_scalars_device_bases(d_values, self.g_lagrange_device[: 8 * n], n)
+
+    def commit_lagrange(self, h_values: np.ndarray) -> np.ndarray:
+        """commitment.rs:456-466: host Lagrange values, resident Lagrange basis."""
+        h = np.ascontiguousarray(h_values, dtype=np.uint64).reshape(-1, 4)
+        return multiexp_device_bases(h, self.g_lagrange_device[: 8 * len(h)])
+
+
+def _scalar_dev(x: int) -> torch.Tensor:
+    return dv.to_device(mont_limbs(x % R_MOD).reshape(1, 4))

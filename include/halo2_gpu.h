@@ -81,58 +81,58 @@ RustError _halo2_multiexp_device_scalars_device_bases(const void* d_scalar, cons
 
 /* ================================================================ NTT */
 
-/* culib.rs:454 ; cuda/src/ntt.cu:383-393 */
+/* culib.rs:492 ; cuda/src/ntt.cu:383-393 */
 uint64_t _halo2_fft_normal_workspace_size(uint32_t ntt_type, uint32_t log_n, uint32_t extend_log_n);
-/* culib.rs:456-466 ; ntt.cu:395-474 — host input/output/omega/divisor; FFT, iFFT, cosetFFT,
+/* culib.rs:494-504 ; ntt.cu:395-474 — host input/output/omega/divisor; FFT, iFFT, cosetFFT,
  * icosetFFT and CosetFFT_Part at size 2^log_n */
 RustError _halo2_fft_normal(uint32_t ntt_type, uint32_t log_n, const void* input, void* output,
     const void* omega, const void* divisor, void* scratch, uint64_t scratch_bytes, void* stream);
-/* culib.rs:468-473 ; ntt.cu:149-164 — always true (deterministic sizing) */
+/* culib.rs:506-511 ; ntt.cu:149-164 — always true (deterministic sizing) */
 bool _halo2_fft_normal_check_memory(uint32_t ntt_type, const void* input, uint32_t log_n,
     uint32_t extend_log_n);
-/* culib.rs:475-479 ; ntt.cu:480-487 */
+/* culib.rs:513-517 ; ntt.cu:480-487 */
 uint64_t _halo2_fft_normal_to_device_workspace_size(uint32_t ntt_type, uint32_t log_n,
     uint32_t extend_log_n);
-/* culib.rs:484-494 ; ntt.cu:509-572 — device input/output (may alias), device omega,
+/* culib.rs:522-532 ; ntt.cu:509-572 — device input/output (may alias), device omega,
  * host divisor; FFT, iFFT and CosetFFT_Part only */
 RustError _halo2_fft_normal_to_device(uint32_t ntt_type, uint32_t log_n, const void* d_input,
     void* d_output, const void* d_omega, const void* h_divisor, void* scratch,
     uint64_t scratch_bytes, void* stream);
-/* culib.rs:496-507 ; ntt.cu:576-639 — host input (2^log_n) -> host output (2^extend_log_n):
+/* culib.rs:534-545 ; ntt.cu:576-639 — host input (2^log_n) -> host output (2^extend_log_n):
  * zero-pad, multiply index i by zeta^(i mod 3), DFT with the extended omega */
 RustError _halo2_cosetfft(uint32_t ntt_type, uint32_t log_n, uint32_t extend_log_n, const void* input,
     void* output, const void* omega, const void* divisor, void* scratch, uint64_t scratch_bytes,
     void* stream);
-/* culib.rs:509 ; ntt.cu:749-760 */
+/* culib.rs:547 ; ntt.cu:749-760 */
 uint64_t _halo2_fft_many_workspace_size(uint32_t ntt_type, uint32_t log_n, uint32_t extend_log_n);
-/* culib.rs:511-515 ; ntt.cu:765-775 */
+/* culib.rs:549-553 ; ntt.cu:765-775 */
 uint64_t _halo2_fft_many_to_device_workspace_size(uint32_t ntt_type, uint32_t log_n,
     uint32_t extend_log_n);
-/* culib.rs:522-534 ; ntt.cu:813-886 — host inputs/outputs, device omega, host divisor */
+/* culib.rs:560-572 ; ntt.cu:813-886 — host inputs/outputs, device omega, host divisor */
 RustError _halo2_fft_many(uint32_t ntt_type, uint32_t num_many, uint32_t log_n, uint32_t extend_log_n,
     const FFITraitObject* input, const FFITraitObject* output, const void* d_omega,
     const FFITraitObject* divisor, void* scratch, uint64_t scratch_bytes, void* stream);
-/* culib.rs:540-553 ; ntt.cu:898-962 — device outputs; inputs host or device per flag */
+/* culib.rs:578-591 ; ntt.cu:898-962 — device outputs; inputs host or device per flag */
 RustError _halo2_fft_many_to_device(uint32_t ntt_type, uint32_t num_many, uint32_t log_n,
     uint32_t extend_log_n, const FFITraitObject* input, const FFITraitObject* output,
     const void* d_omega, const FFITraitObject* divisor, bool input_on_device, void* scratch,
     uint64_t scratch_bytes, void* stream);
-/* culib.rs:555 ; ntt.cu:780-786 — always true */
+/* culib.rs:593 ; ntt.cu:780-786 — always true */
 bool _halo2_fft_many_check_memory(uint32_t ntt_type, uint32_t log_n, uint32_t extend_log_n);
-/* culib.rs:557-562 ; ntt.cu:666-681 — smallest split (log2) whose working set fits, -1 if none */
+/* culib.rs:595-600 ; ntt.cu:666-681 — smallest split (log2) whose working set fits, -1 if none */
 int32_t _halo2_get_fft_split_radix(uint32_t ntt_type, uint32_t log_n, uint32_t extend_log_n,
     uint64_t free_bytes);
 
 /* ================================================ polynomial helpers */
 
-/* culib.rs:402-408 ; cuda/src/polynomial/polynomial.cu:247-267
+/* culib.rs:383-389 ; cuda/src/polynomial/polynomial.cu:247-267
  * d_out[i * num_parts + p] = d_parts[p][i]; d_parts is a device array of device pointers */
 RustError _halo2_extended_from_lagrange_vec_device(void* d_out, const void* d_parts, uint32_t num_parts,
     uint64_t n, void* stream);
-/* culib.rs:376-382 ; polynomial.cu:270-290 — d_poly[i] *= d_t_evals[i % t_len] */
+/* culib.rs:311-317 ; polynomial.cu:270-290 — d_poly[i] *= d_t_evals[i % t_len] */
 RustError _halo2_divide_by_vanishing_poly(void* d_poly, const void* d_t_evals, uint32_t t_len,
     uint64_t n, void* stream);
-/* culib.rs:410-417 ; polynomial.cu:294-318
+/* culib.rs:393-399 ; polynomial.cu:294-318
  * d_a[i] *= d_coset_powers[(i % (len + 1)) - 1] when i % (len + 1) != 0 */
 RustError _halo2_distribute_powers_zeta(void* d_a, const void* d_coset_powers, uint32_t coset_powers_len,
     uint64_t n, void* stream);
@@ -152,25 +152,25 @@ uint64_t _halo2_permute_expression_pair_workspace_size(uint64_t n, uint64_t usab
 RustError _halo2_permute_expression_pair(const void* d_compressed_input, const void* d_compressed_table,
     void* d_permuted_input, void* d_permuted_table, uint64_t n, uint64_t usable_rows, void* scratch,
     uint64_t scratch_bytes, void* stream);
-/* culib.rs:82 ; cuda/src/commitment/commit_product.cu:33-43 */
+/* culib.rs:81 ; cuda/src/commitment/commit_product.cu:33-43 */
 size_t _halo2_commit_product_max_len(size_t length, uint64_t free_bytes);
-/* culib.rs:83 ; commit_product.cu:45-48 */
+/* culib.rs:82 ; commit_product.cu:45-48 */
 uint64_t _halo2_commit_product_workspace_size(uint64_t poly_length);
-/* culib.rs:88-101 ; commit_product.cu:50-130 — host polynomials at [offset, offset + length),
+/* culib.rs:87-100 ; commit_product.cu:50-130 — host polynomials at [offset, offset + length),
  * device beta/gamma: (A'+beta)(S'+gamma) inverted, times (A+beta)(S+gamma) */
 RustError _halo2_commit_product(const FFITraitObject* lookup_product, const FFITraitObject* permuted_input,
     const FFITraitObject* permuted_table, const FFITraitObject* compressed_input,
     const FFITraitObject* compressed_table, const void* beta_device, const void* gamma_device,
     size_t length, size_t offset, void* scratch, uint64_t scratch_bytes, void* stream);
-/* culib.rs:115-126 ; commit_product.cu:142-186 — every .ptr is a device pointer */
+/* culib.rs:114-125 ; commit_product.cu:142-186 — every .ptr is a device pointer */
 RustError _halo2_commit_product_device_inputs(const FFITraitObject* d_lookup_product,
     const FFITraitObject* d_permuted_input, const FFITraitObject* d_permuted_table,
     const FFITraitObject* d_compressed_input, const FFITraitObject* d_compressed_table,
     const void* beta_device, const void* gamma_device, size_t length, size_t offset, void* stream);
 
-/* culib.rs:142 ; cuda/src/commitment/permutation_commit.cu:43-53 */
+/* culib.rs:143 ; cuda/src/commitment/permutation_commit.cu:43-53 */
 size_t _halo2_permutation_product_max_len(size_t length, uint64_t free_bytes);
-/* culib.rs:143 ; permutation_commit.cu:55-58 */
+/* culib.rs:144 ; permutation_commit.cu:55-58 */
 uint64_t _halo2_permutation_product_workspace_size(uint64_t poly_length);
 /* culib.rs:146-161 ; permutation_commit.cu:60-155 — host chunk buffers; modified_values is the
  * running accumulator (length entries); permutations/values are arrays of `batch` host columns
@@ -179,7 +179,7 @@ RustError _halo2_permutation_product(const FFITraitObject* modified_values, cons
     const FFITraitObject* values, const void* beta_device, const void* gamma_device, const void* delta_device,
     const void* omega_device, void* deltaomega_device, size_t length, size_t offset, size_t batch,
     void* scratch, uint64_t scratch_bytes, void* stream);
-/* culib.rs:163 ; permutation_commit.cu:175-178 */
+/* culib.rs:164 ; permutation_commit.cu:175-178 */
 uint64_t _halo2_permutation_product_device_inputs_workspace_size(uint64_t poly_length);
 /* culib.rs:180-195 ; permutation_commit.cu:180-256 — device variant */
 RustError _halo2_permutation_product_device_inputs(const FFITraitObject* modified_values_device,
@@ -187,9 +187,9 @@ RustError _halo2_permutation_product_device_inputs(const FFITraitObject* modifie
     const void* gamma_device, const void* delta_device, const void* omega_device, void* deltaomega_device,
     size_t length, size_t offset, size_t batch, void* scratch, uint64_t scratch_bytes, void* stream);
 
-/* culib.rs:197 ; cuda/src/commitment/grand_product.cu:31-41 */
+/* culib.rs:199 ; cuda/src/commitment/grand_product.cu:31-41 */
 size_t _halo2_grand_product_max_len(size_t length, uint64_t free_bytes);
-/* culib.rs:198 ; grand_product.cu:43-46 */
+/* culib.rs:200 ; grand_product.cu:43-46 */
 uint64_t _halo2_grand_product_workspace_size(uint64_t poly_length);
 /* culib.rs:202-211 ; grand_product.cu:48-123 — output[i] = prefix * prod_{j<=i} input[j] over
  * [offset, offset + length), host buffers, host prefix */
@@ -202,12 +202,12 @@ RustError _halo2_grand_product_device_inputs(const FFITraitObject* d_inout, cons
 
 /* ===================================================== quotient pieces */
 
-/* culib.rs:564-579 ; cuda/src/quotient_lookups.cu:16-53 — all device pointers */
+/* culib.rs:602-616 ; cuda/src/quotient_lookups.cu:16-53 — all device pointers */
 RustError _halo2_quotient_lookups(void* values, const void* table_values, const void* product_coset,
     const void* permuted_input_coset, const void* permuted_table_coset, const void* l0, const void* l_last,
     const void* l_active_row, const void* beta, const void* gamma, const void* y, size_t poly_length,
     void* stream);
-/* culib.rs:585-616 ; cuda/src/permutation_quotient.cu:23-93 — device vectors and pointer tables,
+/* culib.rs:622-650 ; cuda/src/permutation_quotient.cu:23-93 — device vectors and pointer tables,
  * host scalars */
 RustError _halo2_quotient_permutation(void* values, const void* l0, const void* l_last,
     const void* l_active_row, const void* perm_prod_cosets, const void* perm_cosets,

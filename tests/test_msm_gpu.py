@@ -196,3 +196,30 @@ def test_kzg_commit_srs_identity(lib, co):
     m = 1000
     exp_m = co.g1_mul(co.g1_generator(), co.fr_inner_product(poly[:m], pows[:m]))
     assert (params.commit(poly[:m]) == exp_m).all()
+
+
+def test_kzg_commit_lagrange(lib, co):
+    """The Lagrange basis (commitment.rs:157-183): g_lagrange[i] = L_i(tau) G with
+    L_i(tau) = (tau^n - 1)/n * w^i/(tau - w^i) on a sample of i, and commit_lagrange(v)
+    equals commit(iFFT(v)) — the basis is the inverse DFT of the powers basis."""
+    from halo2_amd import device as dv, fields as F, msm
+    from halo2_amd.domain import EvaluationDomain
+
+    k = 14
+    n = 1 << k
+    R = F.R_MOD
+    tau = 0x77AA55 % R
+    params = msm.ParamsKZG(k, tau)
+    w = F.omega(k)
+    gl = dv.to_host(params.g_lagrange_device, cols=8)
+    mult = (pow(tau, n, R) - 1) * pow(n, -1, R) % R
+    idx = [0, 1, 777, n - 1]
+    scal = np.stack([F.mont_limbs(mult * pow(w, i, R) * pow(tau - pow(w, i, R), -1, R) % R) for i in idx])
+    assert (gl[idx] == co.g1_mul_gen(scal)).all()
+    values = rand_fr(co, n, 72)
+    coeffs = EvaluationDomain(3, k).lagrange_to_coeff_device_input(dv.to_device(values))
+    exp = params.commit_device(coeffs)
+    assert (params.commit_lagrange_device(dv.to_device(values)) == exp).all()
+    assert (params.commit_lagrange(values) == exp).all()
+    pows = co.fr_powers(F.mont_limbs(tau), n)
+    assert (exp == co.g1_mul(co.g1_generator(), co.fr_inner_product(dv.to_host(coeffs), pows))).all()
