@@ -286,7 +286,20 @@ def main() -> int:
                 cnt, tot = _lib.profile_read(name)
                 if cnt:
                     extra["msm_2^%d_%s_ms" % (lk, name)] = round(tot / cnt, 3)
-            del sc
+            # the same MSM through the fixed-base table of the resident SRS (built once per SRS)
+            torch.cuda.synchronize()
+            ta = time.perf_counter()
+            table = msm.msm_precompute(srs[: 8 * m], m)
+            torch.cuda.synchronize()
+            extra["msm_fixed_base_table_2^%d_ms" % lk] = round((time.perf_counter() - ta) * 1e3, 1)
+            msm.multiexp_precomputed(sc, table, m, m)  # warm-up
+            ta = time.perf_counter()
+            for _ in range(args.msm_reps):
+                msm.multiexp_precomputed(sc, table, m, m)
+            dt = (time.perf_counter() - ta) / args.msm_reps
+            extra["msm_fixed_base_2^%d_ms" % lk] = round(dt * 1e3, 3)
+            extra["msm_fixed_base_2^%d_elems_per_s" % lk] = m / dt
+            del sc, table
         del srs
 
     # ----------------------------------------- the other hot-path rows

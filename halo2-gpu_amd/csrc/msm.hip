@@ -64,9 +64,11 @@ struct PointSlot {
 struct MsmPlan {
     uint64_t n;
     uint32_t c, W, B;
-    uint64_t nb; // W * B buckets
+    uint32_t Wb; // bucket windows: W, or 1 when the windows share buckets (precomputed bases)
+    uint64_t table_n; // precomputed bases: row stride of the table (0 = plain bases)
+    uint64_t nb; // Wb * B buckets
     uint32_t FB, CB; // round-2 (fine) and round-1 (coarse) bucket bits, FB + CB = c - 1
-    uint32_t NC; // coarse bins = W << CB
+    uint32_t NC; // coarse bins = Wb << CB
     uint32_t ntiles; // round-1 tiles
     uint64_t max_chunks; // upper bound on round-2 chunks
     uint32_t S; // sorted entries per accumulation thread
@@ -85,24 +87,36 @@ uint32_t log2_floor(uint64_t n)
     return l;
 }
 
-MsmPlan make_plan(uint64_t n)
+// Window width: plain bases need W bucket sets (one per window) and a Horner
+// over windows, so c grows more slowly than log2 n; precomputed bases
+// (rows 2^(c w) P) share one bucket set, so the width can follow log2 n.
+uint32_t plan_width(uint64_t n, bool shared)
+{
+    int c = (int)log2_floor(n < 1 ? 1 : n) - (shared ? 0 : 4);
+    if (c < 4)
+        c = 4;
+    if (c > (shared ? 22 : 20))
+        c = shared ? 22 : 20;
+    // same window count, narrowest width: the top window is as full as the others
+    const uint32_t w0 = (255 + (uint32_t)c - 1) / (uint32_t)c;
+    return (255 + w0 - 1) / w0;
+}
+
+// n points; table_n > 0 selects precomputed bases whose table was built for
+// table_n points (the window width follows the table).
+MsmPlan make_plan(uint64_t n, uint64_t table_n = 0)
 {
     MsmPlan p {};
     p.n = n;
-    int c = (int)log2_floor(n < 1 ? 1 : n) - 4;
-    if (c < 4)
-        c = 4;
-    if (c > 20)
-        c = 20;
-    // same window count, narrowest width: the top window is as full as the others
-    const uint32_t w0 = (255 + (uint32_t)c - 1) / (uint32_t)c;
-    p.c = (255 + w0 - 1) / w0;
+    p.table_n = table_n;
+    p.c = plan_width(table_n ? table_n : n, table_n != 0);
     p.W = (255 + p.c - 1) / p.c;
+    p.Wb = table_n ? 1u : p.W;
     p.B = 1u << (p.c - 1);
-    p.nb = (uint64_t)p.W * p.B;
+    p.nb = (uint64_t)p.Wb * p.B;
     p.FB = p.c - 1 < MAX_FB ? p.c - 1 : MAX_FB;
     p.CB = p.c - 1 - p.FB;
-    p.NC = p.W << p.CB;
+    p.NC = p.Wb << p.CB;
     p.ntiles = (uint32_t)((n + R1_TILE - 1) / R1_TILE);
     const uint64_t work = n * p.W;
     p.max_chunks = work / R2_CHUNK + p.NC;
@@ -160,10 +174,10 @@ void carve(WS& ws, const MsmPlan& p, MsmBuffers* b)
     uint64_t scan_n = nh1 + 1;
     if (nh2 + 1 > scan_n)
         scan_n = nh2 + 1;
-    const uint64_t nred = (uint64_t)p.W * p.Mlev[0];
+    const uint64_t nred = (uint64_t)p.Wb * p.Mlev[0];
     void* v[24];
     int k = 0;
-    v[k++] = ws.take(64 * p.n);
+    v[k++] = ws.take(p.table_n ? 0 : 64 * p.n);
     v[k++] = ws.take(32 * p.n);
     v[k++] = ws.take(4 * nh1);
     v[k++] = ws.take(4 * (nh1 + 1));
@@ -213,9 +227,9 @@ struct SizerAdapter {
     }
 };
 
-uint64_t msm_workspace_bytes(uint64_t n)
+uint64_t msm_workspace_bytes(uint64_t n, uint64_t table_n = 0)
 {
-    MsmPlan p = make_plan(n);
+    MsmPlan p = make_plan(n, table_n);
     SizerAdapter sa;
     carve(sa, p, nullptr);
     return sa.s.bytes + 4096;
@@ -272,6 +286,7 @@ __device__ __forceinline__ int32_t next_digit(const uint32_t* s, uint32_t w, uin
 struct SortParams {
     uint64_t n;
     uint32_t c, W, FB, CB, NC, ntiles;
+    uint64_t table_n; // > 0: windows share the bins and entry w of point i indexes row w of the table
 };
 
 // Round 1, pass 1: per tile of R1_TILE points, LDS histogram over the coarse
@@ -298,7 +313,7 @@ __global__ void __launch_bounds__(R1_THREADS) r1_count_kernel(const uint32_t* __
             const int32_t d = next_digit(s, w, sp.c, carry);
             if (d != 0) {
                 const uint32_t bi = (uint32_t)(d < 0 ? -d : d) - 1u;
-                atomicAdd(&hist[(w << sp.CB) + (bi >> sp.FB)], 1u);
+                atomicAdd(&hist[(sp.table_n ? 0u : w << sp.CB) + (bi >> sp.FB)], 1u);
             }
         }
     }
@@ -311,8 +326,9 @@ __global__ void __launch_bounds__(R1_THREADS) r1_count_kernel(const uint32_t* __
 // Window by window the tile's entries are first ordered by coarse bin in LDS
 // (LDS ranks, a scan of the 2^CB bin counts), then written out in that order,
 // so consecutive threads store consecutive addresses of one bin's run instead
-// of isolated 8-byte words. A (window, bin) pair is written in one window
-// iteration only, so the per-tile cursors need no update.
+// of isolated 8-byte words. With a bin per (window, bucket bits) a bin is
+// written in one window iteration only; with shared bins (precomputed bases)
+// the tile's cursors advance after every window.
 uint32_t r1_scatter_lds(uint32_t NC, uint32_t CB)
 {
     return 4 * ((NC + 1) & ~1u) + 4 * (((1u << CB) + 2) & ~1u) + 10 * R1_TILE; // < 56 KiB for c <= 20
@@ -357,7 +373,8 @@ __global__ void __launch_bounds__(R1_THREADS) r1_scatter_kernel(const uint32_t* 
             if (d != 0) {
                 const uint32_t bi = (uint32_t)(d < 0 ? -d : d) - 1u;
                 bin[k] = bi >> sp.FB;
-                ent[k] = ((uint64_t)(bi & fmask) << 32) | (uint32_t)i | (d < 0 ? 0x80000000u : 0u);
+                const uint32_t idx = sp.table_n ? (uint32_t)(w * sp.table_n + i) : (uint32_t)i;
+                ent[k] = ((uint64_t)(bi & fmask) << 32) | idx | (d < 0 ? 0x80000000u : 0u);
                 rank[k] = atomicAdd(&base[bin[k]], 1u);
             }
         }
@@ -403,11 +420,17 @@ __global__ void __launch_bounds__(R1_THREADS) r1_scatter_kernel(const uint32_t* 
         }
         __syncthreads();
         const uint32_t total = base[BW];
+        const uint32_t wbin = sp.table_n ? 0u : w << sp.CB;
         for (uint32_t idx = threadIdx.x; idx < total; idx += R1_THREADS) {
             const uint32_t b = sbin[idx];
-            kv1[cursor[(w << sp.CB) + b] + (idx - base[b])] = stage[idx];
+            kv1[cursor[wbin + b] + (idx - base[b])] = stage[idx];
         }
         __syncthreads();
+        if (sp.table_n) { // shared bins: the next window continues after this one's entries
+            for (uint32_t b = threadIdx.x; b < BW; b += R1_THREADS)
+                cursor[b] += base[b + 1] - base[b];
+            __syncthreads();
+        }
     }
 }
 
@@ -795,6 +818,50 @@ __global__ void __launch_bounds__(64) finalize_kernel(const PointSlot* __restric
     }
 }
 
+// ------------------------------------------------- precomputed window bases
+//
+// Row w of the table holds 2^(c w) P_i (affine, 2^261 scale, 64 B), so the
+// digit of window w of scalar i selects row w and every window shares one
+// bucket set. A row is the previous row doubled c times in XYZZ, then made
+// affine with one batched inversion of ZZZ: 1/Z = ZZ/ZZZ, x = X/Z^2, y = Y/ZZZ.
+
+// xyzz[i] = 2^c * row[i]; zzz[i] = ZZZ as a 32-byte Montgomery word (2^256 scale)
+__global__ void window_double_kernel(const uint8_t* __restrict__ row, PointSlot* __restrict__ xyzz,
+    uint32_t* __restrict__ zzz, uint64_t n, uint32_t c)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    G1X29 acc = G1X29::dbl_affine(G1A29::load(row + 64 * i)); // the identity gives ZZ = ZZZ = 0
+    for (uint32_t k = 1; k < c; ++k)
+        acc = G1X29::dbl_inline(acc);
+    acc.store(xyzz + i);
+    Fq29 from;
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+        from.l[q] = Fq29Params::FROM261[q];
+    Fq29::reduce_full(Fq29::mul(acc.ZZZ, from)).store32(zzz + 8 * i);
+}
+
+// row[i] = affine(xyzz[i]) with inv[i] = 1/ZZZ (2^256 scale words; 0 for the identity)
+__global__ void window_affine_kernel(const PointSlot* __restrict__ xyzz, const uint32_t* __restrict__ inv,
+    uint8_t* __restrict__ row, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    Fq k;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        k.v[q] = Fq29Params::TO261[q];
+    const Fq29 iz3 = Fq29::from_fp(Fq::load(inv + 8 * i) * k); // 1/ZZZ in the 2^261 scale
+    const G1X29 p = G1X29::load(xyzz + i);
+    const Fq29 iz = Fq29::mul(p.ZZ, iz3); // 1/Z
+    const Fq29 iz2 = Fq29::mul(iz, iz);
+    Fq29::reduce_full(Fq29::mul(p.X, iz2)).store32(row + 64 * i);
+    Fq29::reduce_full(Fq29::mul(p.Y, iz3)).store32(row + 64 * i + 32);
+}
+
 unsigned blocks_for(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
 uint32_t ilog2(uint32_t x) { return 31 - __builtin_clz(x); }
@@ -808,9 +875,13 @@ RustError msm_device(const MsmPlan& p, const MsmBuffers& b, const Fr* d_scalars,
     std::unique_ptr<ProfScope> prof_sort(new ProfScope("msm_sort", stream));
     to_canonical_kernel<<<blocks_for(p.n, T), T, 0, stream>>>(d_scalars, b.canon, p.n);
     H2_LAUNCH_CHECK();
-    to_scale261_kernel<<<blocks_for(p.n, T), T, 0, stream>>>(d_points, b.pts, p.n);
-    H2_LAUNCH_CHECK();
-    SortParams sp { p.n, p.c, p.W, p.FB, p.CB, p.NC, p.ntiles };
+    const uint8_t* pts = d_points; // a precomputed table is already in the 2^261 scale
+    if (!p.table_n) {
+        to_scale261_kernel<<<blocks_for(p.n, T), T, 0, stream>>>(d_points, b.pts, p.n);
+        H2_LAUNCH_CHECK();
+        pts = b.pts;
+    }
+    SortParams sp { p.n, p.c, p.W, p.FB, p.CB, p.NC, p.ntiles, p.table_n };
     const uint64_t nh1 = (uint64_t)p.NC * p.ntiles;
     r1_count_kernel<<<p.ntiles, R1_THREADS, 4 * p.NC, stream>>>(b.canon, sp, b.h1);
     H2_LAUNCH_CHECK();
@@ -833,7 +904,7 @@ RustError msm_device(const MsmPlan& p, const MsmBuffers& b, const Fr* d_scalars,
 
     {
         ProfScope prof("msm_buckets", stream);
-        accumulate_kernel<<<blocks_for(p.nthr, T), T, 0, stream>>>(b.entries, b.pts, b.off0, p.nb, p.S, b.part,
+        accumulate_kernel<<<blocks_for(p.nthr, T), T, 0, stream>>>(b.entries, pts, b.off0, p.nb, p.S, b.part,
             b.bsum);
         H2_LAUNCH_CHECK();
         H2_TRY(hipMemsetAsync(b.heavy + p.nb, 0, 4, stream));
@@ -845,19 +916,19 @@ RustError msm_device(const MsmPlan& p, const MsmBuffers& b, const Fr* d_scalars,
 
     ProfScope prof_red("msm_reduce", stream);
     const uint64_t M0 = p.Mlev[0];
-    reduce_first_kernel<<<blocks_for((uint64_t)p.W * M0, T), T, 0, stream>>>(b.bsum, b.off0, p.W, p.B, p.L1,
+    reduce_first_kernel<<<blocks_for((uint64_t)p.Wb * M0, T), T, 0, stream>>>(b.bsum, b.off0, p.Wb, p.B, p.L1,
         ilog2(p.L1), M0, b.red[0], b.red[1]);
     H2_LAUNCH_CHECK();
     int cur = 0;
     for (uint32_t lev = 1; lev <= p.nlev; ++lev) {
         const uint64_t Mi = p.Mlev[lev - 1];
         const uint64_t Mo = p.Mlev[lev];
-        reduce_tree_kernel<<<(unsigned)(p.W * Mo), TREE, 0, stream>>>(b.red[2 * cur], b.red[2 * cur + 1], Mi,
+        reduce_tree_kernel<<<(unsigned)(p.Wb * Mo), TREE, 0, stream>>>(b.red[2 * cur], b.red[2 * cur + 1], Mi,
             p.Klev[lev], ilog2(p.Klev[lev]), b.red[2 * (cur ^ 1)], b.red[2 * (cur ^ 1) + 1]);
         H2_LAUNCH_CHECK();
         cur ^= 1;
     }
-    finalize_kernel<<<1, 64, 0, stream>>>(b.red[2 * cur], p.W, p.c, b.out);
+    finalize_kernel<<<1, 64, 0, stream>>>(b.red[2 * cur], p.Wb, p.c, b.out);
     H2_LAUNCH_CHECK();
     return ok();
 }
@@ -946,6 +1017,83 @@ RustError _halo2_multiexp_device_scalars_device_bases(const void* d_scalar, cons
 {
     return msm_entry("_halo2_multiexp_device_scalars_device_bases", d_scalar, true, d_bases, true, h_output->ptr,
         length, scratch, scratch_bytes, (hipStream_t)stream);
+}
+
+// ----------------------------------------- fixed bases (no reference counterpart)
+
+uint64_t h2amd_msm_table_bytes(uint64_t n)
+{
+    const MsmPlan p = make_plan(n, n ? n : 1);
+    return 64ull * n * p.W;
+}
+
+uint64_t h2amd_msm_precompute_workspace_size(uint64_t n) { return 160 * n + 512; }
+
+RustError h2amd_msm_precompute(void* d_table, const void* d_bases, uint64_t n, void* scratch, uint64_t scratch_bytes,
+    void* stream_)
+{
+    hipStream_t stream = (hipStream_t)stream_;
+    if (n == 0)
+        return ok();
+    H2_CHECK_ARG(n < (1ull << 31), "h2amd_msm_precompute: length %lu exceeds 2^31", (unsigned long)n);
+    const MsmPlan p = make_plan(n, n);
+    H2_CHECK_ARG(n * p.W < (1ull << 31), "h2amd_msm_precompute: %lu points x %u windows exceed 2^31 entries",
+        (unsigned long)n, p.W);
+    Workspace ws(scratch, scratch_bytes);
+    PointSlot* xyzz = (PointSlot*)ws.take(128 * n);
+    uint32_t* zzz = (uint32_t*)ws.take(32 * n);
+    if (ws.overflow)
+        return fail((int)hipErrorInvalidValue, "h2amd_msm_precompute: workspace of %lu bytes is too small",
+            (unsigned long)scratch_bytes);
+    uint8_t* table = (uint8_t*)d_table;
+    const unsigned T = 256;
+    to_scale261_kernel<<<blocks_for(n, T), T, 0, stream>>>((const uint8_t*)d_bases, table, n);
+    H2_LAUNCH_CHECK();
+    for (uint32_t w = 1; w < p.W; ++w) {
+        window_double_kernel<<<blocks_for(n, T), T, 0, stream>>>(table + 64 * n * (w - 1), xyzz, zzz, n, p.c);
+        H2_LAUNCH_CHECK();
+        RustError e = _halo2_batch_invert(zzz, 1, n, stream);
+        if (e.code)
+            return e;
+        window_affine_kernel<<<blocks_for(n, T), T, 0, stream>>>(xyzz, zzz, table + 64 * n * w, n);
+        H2_LAUNCH_CHECK();
+    }
+    return ok();
+}
+
+uint64_t h2amd_multiexp_precomputed_workspace_size(uint64_t length, uint64_t table_n)
+{
+    return msm_workspace_bytes(length, table_n ? table_n : 1);
+}
+
+RustError h2amd_multiexp_precomputed(const void* d_scalar, const void* d_table, uint64_t table_n,
+    const FFITraitObject* h_output, uint64_t length, void* scratch, uint64_t scratch_bytes, void* stream_)
+{
+    hipStream_t stream = (hipStream_t)stream_;
+    uint64_t* o = (uint64_t*)h_output->ptr;
+    if (length == 0) {
+        memset(o, 0, 96);
+        Fq one = Fq::one();
+        memcpy(o + 4, one.v, 32);
+        return ok();
+    }
+    H2_CHECK_ARG(length <= table_n, "h2amd_multiexp_precomputed: length %lu exceeds the table's %lu points",
+        (unsigned long)length, (unsigned long)table_n);
+    const MsmPlan p = make_plan(length, table_n);
+    H2_CHECK_ARG(table_n * p.W < (1ull << 31), "h2amd_multiexp_precomputed: table too large");
+    Workspace ws(scratch, scratch_bytes);
+    MsmBuffers b {};
+    carve(ws, p, &b);
+    if (ws.overflow)
+        return fail((int)hipErrorInvalidValue,
+            "h2amd_multiexp_precomputed: workspace of %lu bytes is too small for length %lu",
+            (unsigned long)scratch_bytes, (unsigned long)length);
+    RustError e = msm_device(p, b, (const Fr*)d_scalar, (const uint8_t*)d_table, stream);
+    if (e.code)
+        return e;
+    H2_TRY(hipMemcpyAsync(o, b.out, 96, hipMemcpyDeviceToHost, stream));
+    H2_TRY(hipStreamSynchronize(stream));
+    return ok();
 }
 
 } // extern "C"

@@ -135,6 +135,11 @@ SIGNATURES = {
     "_halo2_kate_division_device": (RustError, [P, P, P, U64, P, U64, P]),
     "_halo2_kate_division_device_padded": (RustError, [P, P, P, U64, U64, P, U64, P]),
     "_halo2_decode_assigned": (RustError, [P, P, P, U64, AssignedLayout, P]),
+    "h2amd_msm_table_bytes": (U64, [U64]),
+    "h2amd_msm_precompute_workspace_size": (U64, [U64]),
+    "h2amd_msm_precompute": (RustError, [P, P, U64, P, U64, P]),
+    "h2amd_multiexp_precomputed_workspace_size": (U64, [U64, U64]),
+    "h2amd_multiexp_precomputed": (RustError, [P, P, U64, OBJP, U64, P, U64, P]),
     "h2amd_random_fr": (RustError, [P, U64, U64, U64, P]),
     "h2amd_g1_mul_generator_workspace_size": (U64, [U64]),
     "h2amd_g1_mul_generator": (RustError, [P, P, U64, P, U64, P]),

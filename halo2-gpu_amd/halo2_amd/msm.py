@@ -98,6 +98,28 @@ def best_multiexp(h_scalars: np.ndarray, h_points: np.ndarray) -> np.ndarray:
     return g1.to_out12(acc)
 
 
+def msm_precompute(d_bases: torch.Tensor, n: int) -> torch.Tensor:
+    """Fixed-base table of n device bases (`h2amd_msm_precompute`): W rows of 2^(c w) P_i."""
+    lib = _lib.load()
+    table = torch.empty(max(1, int(lib.h2amd_msm_table_bytes(n)) // 8), dtype=torch.int64, device=d_bases.device)
+    ws = int(lib.h2amd_msm_precompute_workspace_size(n))
+    scratch = dv.empty_bytes(ws)
+    _lib.check(lib.h2amd_msm_precompute(dv.ptr(table), dv.ptr(d_bases), n, dv.ptr(scratch), ws, dv.stream()),
+               "h2amd_msm_precompute")
+    return table
+
+
+def multiexp_precomputed(d_scalars: torch.Tensor, table: torch.Tensor, table_n: int, n: int) -> np.ndarray:
+    """Fixed-base MSM over the first n <= table_n points (`h2amd_multiexp_precomputed`)."""
+    lib = _lib.load()
+    ws = int(lib.h2amd_multiexp_precomputed_workspace_size(n, table_n))
+    scratch = dv.empty_bytes(ws)
+    out, obj = _out_obj()
+    _lib.check(lib.h2amd_multiexp_precomputed(dv.ptr(d_scalars), dv.ptr(table), table_n, ctypes.byref(obj), n,
+                                              dv.ptr(scratch), ws, dv.stream()), "h2amd_multiexp_precomputed")
+    return out
+
+
 def g1_mul_generator(d_scalars: torch.Tensor, n: int) -> torch.Tensor:
     """d_out[i] = s_i * G (affine, 8 words per point) on the device."""
     lib = _lib.load()
@@ -134,7 +156,7 @@ class ParamsKZG:
     (`g_device`, `g_lagrange_device`, :246-304) are the resident buffers themselves.
     G2 elements belong to the verifier and are not part of this path."""
 
-    def __init__(self, k: int, tau: int, omega: int | None = None):
+    def __init__(self, k: int, tau: int, omega: int | None = None, fixed_base: bool = True):
         from .fields import omega as root
 
         self.k = k
@@ -143,6 +165,19 @@ class ParamsKZG:
         self.omega = root(k) if omega is None else omega % R_MOD
         self.g_device = g1_mul_generator(omega_powers(self.tau, k), self.n)
         self._g_lagrange_device = None
+        # fixed-base tables (rows 2^(c w) of each basis), built on first commit
+        self.fixed_base = fixed_base
+        self._tables: dict[str, torch.Tensor] = {}
+
+    def _commit_resident(self, d_scalars: torch.Tensor, which: str) -> np.ndarray:
+        n = d_scalars.numel() // 4
+        assert n <= self.n
+        bases = self.g_device if which == "g" else self.g_lagrange_device
+        if not self.fixed_base:
+            return multiexp_device_scalars_device_bases(d_scalars, bases[: 8 * n], n)
+        if which not in self._tables:
+            self._tables[which] = msm_precompute(bases, self.n)
+        return multiexp_precomputed(d_scalars, self._tables[which], self.n, n)
 
     @property
     def g_lagrange_device(self) -> torch.Tensor:
@@ -164,26 +199,27 @@ class ParamsKZG:
         return self._g_lagrange_device
 
     def commit_device(self, d_poly: torch.Tensor) -> np.ndarray:
-        """commitment.rs:560-577: MSM of device coefficients against the resident SRS."""
-        n = d_poly.numel() // 4
-        assert n <= self.n
-        return multiexp_device_scalars_device_bases(d_poly, self.g_device[: 8 * n], n)
+        """commitment.rs:560-577: MSM of device coefficients against the resident SRS
+        (through its fixed-base table unless fixed_base is off)."""
+        return self._commit_resident(d_poly, "g")
 
     def commit(self, h_poly: np.ndarray) -> np.ndarray:
         """commitment.rs:531-541: host coefficients, resident SRS."""
         h = np.ascontiguousarray(h_poly, dtype=np.uint64).reshape(-1, 4)
+        if self.fixed_base:
+            return self._commit_resident(dv.to_device(h), "g")
         return multiexp_device_bases(h, self.g_device[: 8 * len(h)])
 
     def commit_lagrange_device(self, d_values: torch.Tensor) -> np.ndarray:
         """commitment.rs:467-485: MSM of device Lagrange values against the resident
         Lagrange basis."""
-        n = d_values.numel() // 4
-        assert n <= self.n
-        return multiexp_device_scalars_device_bases(d_values, self.g_lagrange_device[: 8 * n], n)
+        return self._commit_resident(d_values, "g_lagrange")
 
     def commit_lagrange(self, h_values: np.ndarray) -> np.ndarray:
         """commitment.rs:456-466: host Lagrange values, resident Lagrange basis."""
         h = np.ascontiguousarray(h_values, dtype=np.uint64).reshape(-1, 4)
+        if self.fixed_base:
+            return self._commit_resident(dv.to_device(h), "g_lagrange")
         return multiexp_device_bases(h, self.g_lagrange_device[: 8 * len(h)])
 
 

@@ -378,6 +378,24 @@ uint64_t h2amd_g1_mul_generator_workspace_size(uint64_t n);
 RustError h2amd_g1_mul_generator(void* d_out, const void* d_scalars, uint64_t n, void* scratch,
     uint64_t scratch_bytes, void* stream);
 
+/* ================== fixed-base MSM (no reference counterpart)
+ * For bases that stay resident (the KZG SRS): a table of rows 2^(c w) P_i (affine,
+ * internal 2^261 scale, 64 B per point, W rows of n points) lets every window share
+ * one bucket set, removing the per-window bucket reduction and the Horner over
+ * windows. Results equal `_halo2_multiexp_device_scalars_device_bases` bit for bit. */
+
+/* bytes of the table for n bases (W(n) rows of n points) */
+uint64_t h2amd_msm_table_bytes(uint64_t n);
+uint64_t h2amd_msm_precompute_workspace_size(uint64_t n);
+/* d_table (h2amd_msm_table_bytes(n) bytes) from n device affine bases */
+RustError h2amd_msm_precompute(void* d_table, const void* d_bases, uint64_t n, void* scratch,
+    uint64_t scratch_bytes, void* stream);
+uint64_t h2amd_multiexp_precomputed_workspace_size(uint64_t length, uint64_t table_n);
+/* sum_i s_i P_i over the first `length` <= table_n points of a table built for table_n
+ * points; device scalars, host 96-byte output (as `_halo2_multiexp_*`). Synchronises. */
+RustError h2amd_multiexp_precomputed(const void* d_scalar, const void* d_table, uint64_t table_n,
+    const FFITraitObject* h_output, uint64_t length, void* scratch, uint64_t scratch_bytes, void* stream);
+
 /* ======================= kernel timing (no reference counterpart)
  * When enabled, launchers bracket their hot kernels ("ntt_pass", "msm_bucket_level0",
  * ...) with HIP events on the launch stream; reading syncs the recorded events. */

@@ -223,3 +223,61 @@ def test_kzg_commit_lagrange(lib, co):
     assert (params.commit_lagrange(values) == exp).all()
     pows = co.fr_powers(F.mont_limbs(tau), n)
     assert (exp == co.g1_mul(co.g1_generator(), co.fr_inner_product(dv.to_host(coeffs), pows))).all()
+
+
+def _plan_width(n):
+    """Window width of the fixed-base table for n bases: clamp(log2 n, 4, 22), then the
+    narrowest width with the same window count."""
+    c = max(4, min(22, n.bit_length() - 1))
+    w0 = -(-255 // c)
+    return -(-255 // w0)
+
+
+@pytest.mark.parametrize("n", [1, 2, 17, 1000, 1 << 16, (1 << 20) + 3])
+def test_msm_fixed_base_table(lib, co, n):
+    """The fixed-base path equals the plain device MSM bit for bit: full length, prefixes
+    of the table, identity bases and edge scalars; table rows are 2^(c w) P."""
+    from halo2_amd import device as dv, fields as F, msm
+
+    print(f"fixed-base n={n}", flush=True)
+    pts = msm.g1_mul_generator(msm.random_fr(n, seed=90 + n % 1000), n)
+    if n >= 17:  # identity bases (0, 0)
+        pts.view(-1, 8)[3].zero_()
+        pts.view(-1, 8)[n - 1].zero_()
+    sc = rand_fr(co, n, 91)
+    if n >= 17:
+        sc[5] = 0
+        sc[6] = F.mont_limbs(F.R_MOD - 1)
+        sc[7] = F.mont_limbs(1)
+    d_sc = dv.to_device(sc)
+    table = msm.msm_precompute(pts, n)
+    for m in sorted({n, max(1, n // 3), max(1, n - 1)}):
+        exp = msm.multiexp_device_scalars_device_bases(d_sc[: 4 * m], pts[: 8 * m], m)
+        got = msm.multiexp_precomputed(d_sc[: 4 * m], table, n, m)
+        assert (got == exp).all(), m
+    if n <= 1000:
+        c = _plan_width(n)
+        W = -(-255 // c)
+        rows = dv.to_host(table, cols=8)
+        base = dv.to_host(pts, cols=8)
+        i = n // 2
+        for w in (1, W - 1):
+            s = F.mont_limbs(pow(2, c * w, F.R_MOD))
+            exp_pt = co.g1_mul(base[i], s)
+            # the table holds the 2^261-scale form: compare through the oracle's affine bytes
+            got_aff = rows[w * n + i]
+            conv = lambda v: F.mont_limbs(F.value_of(v, F.P_MOD) * pow(2, -5, F.P_MOD) % F.P_MOD, F.P_MOD)  # noqa: E731
+            assert (np.concatenate([conv(got_aff[:4]), conv(got_aff[4:])]) == exp_pt[:8]).all(), w
+
+
+def test_kzg_fixed_base_commit(lib, co):
+    """ParamsKZG commits through the fixed-base tables (default) and the plain path agree."""
+    from halo2_amd import device as dv, msm
+
+    k = 12
+    params = msm.ParamsKZG(k, 0xABCDEF)
+    plain = msm.ParamsKZG(k, 0xABCDEF, fixed_base=False)
+    poly = rand_fr(co, 1 << k, 93)
+    assert (params.commit_device(dv.to_device(poly)) == plain.commit_device(dv.to_device(poly))).all()
+    assert (params.commit(poly[:100]) == plain.commit(poly[:100])).all()
+    assert (params.commit_lagrange(poly) == plain.commit_lagrange(poly)).all()
