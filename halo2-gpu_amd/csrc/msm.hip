@@ -555,8 +555,16 @@ __global__ void __launch_bounds__(R2_THREADS) r2_scatter_kernel(const uint64_t* 
             off0[((uint64_t)b << sp.FB) + low] = h2off[base + (uint64_t)low * kb];
     }
     __syncthreads();
-    for (uint32_t e = start + threadIdx.x; e < end; e += R2_THREADS)
-        atomicAdd(&cnt[(uint32_t)(kv1[e] >> 32)], 1u);
+    // the chunk stays in registers between the histogram and the placement
+    constexpr uint32_t PER_T = R2_CHUNK / R2_THREADS;
+    uint64_t kv[PER_T];
+#pragma unroll
+    for (uint32_t q = 0; q < PER_T; ++q) {
+        const uint32_t e = start + q * R2_THREADS + threadIdx.x;
+        kv[q] = e < end ? kv1[e] : 0;
+        if (e < end)
+            atomicAdd(&cnt[(uint32_t)(kv[q] >> 32)], 1u);
+    }
     __syncthreads();
     // exclusive scan of cnt into loc: FBN / R2_THREADS consecutive keys per thread
     const uint32_t per = (FBN + R2_THREADS - 1) / R2_THREADS;
@@ -584,18 +592,23 @@ __global__ void __launch_bounds__(R2_THREADS) r2_scatter_kernel(const uint64_t* 
         }
     }
     __syncthreads();
-    for (uint32_t e = start + threadIdx.x; e < end; e += R2_THREADS) {
-        const uint64_t kv = kv1[e];
-        const uint32_t low = (uint32_t)(kv >> 32);
-        const uint32_t r = atomicAdd(&cnt[low], 1u);
-        sval[r] = (uint32_t)kv;
-        skey[r] = (uint16_t)low;
+#pragma unroll
+    for (uint32_t q = 0; q < PER_T; ++q) {
+        if (start + q * R2_THREADS + threadIdx.x < end) {
+            const uint32_t low = (uint32_t)(kv[q] >> 32);
+            const uint32_t r = atomicAdd(&cnt[low], 1u);
+            sval[r] = (uint32_t)kv[q];
+            skey[r] = (uint16_t)low;
+        }
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < end - start; i += R2_THREADS) {
-        const uint32_t low = skey[i];
-        entries[h2off[base + (uint64_t)low * kb + j] + (i - loc[low])] = sval[i];
-    }
+    // cnt is free again: it now holds, per low key, the global start of this
+    // chunk's run minus its local start
+    for (uint32_t low = threadIdx.x; low < FBN; low += R2_THREADS)
+        cnt[low] = h2off[base + (uint64_t)low * kb + j] - loc[low];
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < end - start; i += R2_THREADS)
+        entries[cnt[skey[i]] + i] = sval[i];
 }
 
 // Thread t adds the sorted entries [tS, min(tS + S, E)). A bucket wholly in
