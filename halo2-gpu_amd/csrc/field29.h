@@ -250,7 +250,8 @@ struct F29 {
     // a^(m - 2) with Montgomery products (2-bit fixed windows, a small register
     // footprint: it runs in one thread of kernels that keep many values live): for
     // a = u * 2^261 this is u^-1 * 2^261; zero maps to zero. Input normalised, < 2m.
-    __device__ static F29 pow_m_minus_2(const F29& a)
+    // Inlined, with the operand by value (callers pass LDS-resident values).
+    __device__ __forceinline__ static F29 pow_m_minus_2(const F29 a)
     {
         const F29 a2 = mul(a, a);
         const F29 a3 = mul(a2, a);

@@ -103,6 +103,91 @@ __global__ void __launch_bounds__(T) batch_invert_kernel(uint8_t* __restrict__ d
     }
 }
 
+// ------------------------------------- with a workspace: few exponentiations
+//
+// With more than one work-group, the per-group exponentiation above (a chain of
+// ~380 dependent products on one lane) dominates. Callers that hold a
+// workspace take three steps instead: (1) every group writes the product of its
+// elements to tot[g]; (2) the totals are inverted by the kernel above (one
+// exponentiation per 2048 totals, in parallel groups); (3) every group finishes
+// with its inverted total. The values match the single-kernel path bit for bit.
+
+// (1) tot[g] = product of the non-zero elements of group g (2^261-scale word)
+template <class P>
+__global__ void __launch_bounds__(T) group_total_kernel(const uint8_t* __restrict__ data, uint64_t n,
+    uint8_t* __restrict__ tot)
+{
+    using F = F29<P>;
+    __shared__ F red[T];
+    const unsigned tid = threadIdx.x;
+    const uint64_t base = (uint64_t)blockIdx.x * C;
+    F acc = F::one();
+#pragma unroll
+    for (unsigned k = 0; k < E; ++k) {
+        const uint64_t i = base + (uint64_t)k * T + tid;
+        if (i < n) {
+            const F a = F::load32(data + 32 * i);
+            if (!a.is_zero())
+                acc = F::mul(acc, a);
+        }
+    }
+    red[tid] = acc;
+    for (unsigned d = T / 2; d > 0; d >>= 1) {
+        __syncthreads();
+        if (tid < d)
+            red[tid] = F::mul(red[tid], red[tid + d]);
+    }
+    __syncthreads();
+    if (tid == 0)
+        F::reduce_full(red[0]).store32(tot + 32ull * blockIdx.x);
+}
+
+// (3) as batch_invert_kernel, with the group's inverse total read from inv[g]:
+// the kernel above turns the word t * 2^261 into t^-1 * 2^-10 in the 2^261 scale,
+// which is exactly the single-group kernel's inv_total.
+template <class P>
+__global__ void __launch_bounds__(T) group_finish_kernel(uint8_t* __restrict__ data, uint64_t n,
+    const uint8_t* __restrict__ inv_tot)
+{
+    using F = F29<P>;
+    __shared__ F pre[T];
+    __shared__ F suf[T];
+    const unsigned tid = threadIdx.x;
+    const uint64_t base = (uint64_t)blockIdx.x * C;
+    F part[E];
+    F acc = F::one();
+#pragma unroll
+    for (unsigned k = 0; k < E; ++k) {
+        const uint64_t i = base + (uint64_t)k * T + tid;
+        part[k] = acc;
+        if (i < n) {
+            const F a = F::load32(data + 32 * i);
+            if (!a.is_zero())
+                acc = F::mul(acc, a);
+        }
+    }
+    pre[tid] = acc;
+    suf[tid] = acc;
+    block_scan_inclusive(pre, tid, false);
+    block_scan_inclusive(suf, tid, true);
+    F inv = F::load32(inv_tot + 32ull * blockIdx.x);
+    if (tid > 0)
+        inv = F::mul(inv, pre[tid - 1]);
+    if (tid + 1 < T)
+        inv = F::mul(inv, suf[tid + 1]);
+#pragma unroll
+    for (int k = (int)E - 1; k >= 0; --k) {
+        const uint64_t i = base + (uint64_t)k * T + tid;
+        if (i < n) {
+            const F a = F::load32(data + 32 * i);
+            if (!a.is_zero()) {
+                F::reduce_full(F::mul(inv, part[k])).store32(data + 32 * i);
+                inv = F::mul(inv, a);
+            }
+        }
+    }
+}
+
 template <class F>
 struct Form29;
 template <>
@@ -114,14 +199,33 @@ struct Form29<Fq> {
     using P = Fq29P;
 };
 
-template <class F>
-RustError batch_invert(F* data, uint64_t n, hipStream_t stream)
+// bytes of workspace batch_invert uses for n elements (0 for one group)
+uint64_t batch_invert_ws_bytes(uint64_t n)
 {
+    const uint64_t groups = (n + C - 1) / C;
+    return groups > 1 ? align_up(32 * groups, 256) : 0;
+}
+
+// In place; with `ws` of batch_invert_ws_bytes(n) bytes the three-step path runs.
+template <class F>
+RustError batch_invert(F* data, uint64_t n, hipStream_t stream, void* ws = nullptr, uint64_t ws_bytes = 0)
+{
+    using P = typename Form29<F>::P;
     if (n == 0)
         return ok();
-    const uint64_t blocks = (n + C - 1) / C;
-    H2_CHECK_ARG(blocks < (1ull << 31), "batch invert of %lu elements is too large", (unsigned long)n);
-    batch_invert_kernel<typename Form29<F>::P><<<(unsigned)blocks, T, 0, stream>>>((uint8_t*)data, n);
+    const uint64_t groups = (n + C - 1) / C;
+    H2_CHECK_ARG(groups < (1ull << 31), "batch invert of %lu elements is too large", (unsigned long)n);
+    if (groups == 1 || ws == nullptr || ws_bytes < 32 * groups) {
+        batch_invert_kernel<P><<<(unsigned)groups, T, 0, stream>>>((uint8_t*)data, n);
+        H2_LAUNCH_CHECK();
+        return ok();
+    }
+    uint8_t* tot = (uint8_t*)ws;
+    group_total_kernel<P><<<(unsigned)groups, T, 0, stream>>>((const uint8_t*)data, n, tot);
+    H2_LAUNCH_CHECK();
+    batch_invert_kernel<P><<<(unsigned)((groups + C - 1) / C), T, 0, stream>>>(tot, groups);
+    H2_LAUNCH_CHECK();
+    group_finish_kernel<P><<<(unsigned)groups, T, 0, stream>>>((uint8_t*)data, n, tot);
     H2_LAUNCH_CHECK();
     return ok();
 }
@@ -217,26 +321,41 @@ __global__ void perm_denominator_kernel(Fr* __restrict__ den, const Fr* __restri
     }
 }
 
+// omega^i = lo[i mod 2^kSplit] * hi[i >> kSplit]: the two tables (2^kSplit and
+// ceil(n / 2^kSplit) entries) stay in L2, so no thread raises omega to a power
+constexpr uint32_t kSplit = 11;
+
+uint64_t split_hi(uint64_t n) { return (n + (1ull << kSplit) - 1) >> kSplit; }
+
+uint64_t split_bytes(uint64_t n) { return align_up(32ull << kSplit, 256) + align_up(32 * split_hi(n), 256); }
+
+__global__ void split_powers_kernel(Fr* __restrict__ lo, Fr* __restrict__ hi, const Fr* __restrict__ d_omega,
+    uint64_t nhi)
+{
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const Fr w = Fr::load(d_omega);
+    if (j < (1u << kSplit))
+        Fr::pow_u64(w, j).store(lo + j);
+    if (j < nhi)
+        Fr::pow_u64(w, j << kSplit).store(hi + j);
+}
+
 // num[i] (*)= deltaomega * omega^i * beta + gamma + value[i]   (permutation_commit.h:40-75)
 // `first` initialises num instead of multiplying into it.
 __global__ void perm_numerator_kernel(Fr* __restrict__ num, const Fr* __restrict__ val, const Fr* __restrict__ d_beta,
-    const Fr* __restrict__ d_gamma, const Fr* __restrict__ d_omega, const Fr* __restrict__ d_deltaomega, uint64_t n,
-    int first)
+    const Fr* __restrict__ d_gamma, const Fr* __restrict__ lo, const Fr* __restrict__ hi,
+    const Fr* __restrict__ d_deltaomega, uint64_t n, int first)
 {
-    const Fr beta = Fr::load(d_beta), gamma = Fr::load(d_gamma), w = Fr::load(d_omega);
-    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const Fr gamma = Fr::load(d_gamma);
+    const Fr c = Fr::load(d_deltaomega) * Fr::load(d_beta);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    if (tid >= n)
-        return;
-    Fr dw = Fr::load(d_deltaomega) * Fr::pow_u64(w, tid);
-    const Fr wstep = Fr::pow_u64(w, stride);
-    for (uint64_t i = tid; i < n; i += stride) {
-        Fr t = dw * beta + gamma + Fr::load(val + i);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const Fr w = Fr::load(lo + (i & ((1u << kSplit) - 1))) * Fr::load(hi + (i >> kSplit));
+        Fr t = c * w + gamma + Fr::load(val + i);
         if (first)
             t.store(num + i);
         else
             (Fr::load(num + i) * t).store(num + i);
-        dw = dw * wstep;
     }
 }
 
@@ -277,8 +396,9 @@ __global__ void lookup_numerator_kernel(Fr* __restrict__ out, const Fr* __restri
 
 // Folds `batch` columns into the denominator and numerator accumulators; the
 // first column of a call with `init_num` initialises the numerator.
+// `lo`/`hi` hold the split omega powers (split_powers_kernel) for this call.
 RustError permutation_product_columns(Fr* den, Fr* num, const FFITraitObject* perms, const FFITraitObject* values,
-    const Fr* d_beta, const Fr* d_gamma, const Fr* d_delta, const Fr* d_omega, Fr* d_deltaomega, uint64_t n,
+    const Fr* d_beta, const Fr* d_gamma, const Fr* d_delta, const Fr* lo, const Fr* hi, Fr* d_deltaomega, uint64_t n,
     uint64_t offset, uint64_t batch, bool init_num, hipStream_t stream)
 {
     for (uint64_t c = 0; c < batch; ++c) {
@@ -286,7 +406,7 @@ RustError permutation_product_columns(Fr* den, Fr* num, const FFITraitObject* pe
         const Fr* val = (const Fr*)values[c].ptr + offset;
         perm_denominator_kernel<<<ew_grid(n), kEwBlock, 0, stream>>>(den, perm, val, d_beta, d_gamma, n);
         H2_LAUNCH_CHECK();
-        perm_numerator_kernel<<<ew_grid(n), kEwBlock, 0, stream>>>(num, val, d_beta, d_gamma, d_omega, d_deltaomega,
+        perm_numerator_kernel<<<ew_grid(n), kEwBlock, 0, stream>>>(num, val, d_beta, d_gamma, lo, hi, d_deltaomega,
             n, init_num && c == 0);
         H2_LAUNCH_CHECK();
         scalar_mul_kernel<<<1, 64, 0, stream>>>(d_deltaomega, d_delta);
@@ -295,10 +415,10 @@ RustError permutation_product_columns(Fr* den, Fr* num, const FFITraitObject* pe
     return ok();
 }
 
-// den <- num / den
-RustError permutation_product_finish(Fr* den, Fr* num, uint64_t n, hipStream_t stream)
+// den <- num / den; `inv_ws` holds batch_invert_ws_bytes(n) bytes (or is null)
+RustError permutation_product_finish(Fr* den, Fr* num, uint64_t n, void* inv_ws, hipStream_t stream)
 {
-    RustError e = batch_invert<Fr>(den, n, stream);
+    RustError e = batch_invert<Fr>(den, n, stream, inv_ws, batch_invert_ws_bytes(n));
     if (e.code)
         return e;
     mul_into_kernel<<<ew_grid(n), kEwBlock, 0, stream>>>(den, num, n);
@@ -306,16 +426,28 @@ RustError permutation_product_finish(Fr* den, Fr* num, uint64_t n, hipStream_t s
     return ok();
 }
 
-RustError permutation_product_device(Fr* den, Fr* num, const FFITraitObject* perms, const FFITraitObject* values,
-    const Fr* d_beta, const Fr* d_gamma, const Fr* d_delta, const Fr* d_omega, Fr* d_deltaomega, uint64_t n,
-    uint64_t offset, uint64_t batch, hipStream_t stream)
+RustError split_powers(Fr* lo, Fr* hi, const Fr* d_omega, uint64_t n, hipStream_t stream)
+{
+    const uint64_t nhi = split_hi(n);
+    const uint64_t m = nhi > (1u << kSplit) ? nhi : (1u << kSplit);
+    split_powers_kernel<<<(unsigned)((m + 255) / 256), 256, 0, stream>>>(lo, hi, d_omega, nhi);
+    H2_LAUNCH_CHECK();
+    return ok();
+}
+
+RustError permutation_product_device(Fr* den, Fr* num, Fr* lo, Fr* hi, void* inv_ws, const FFITraitObject* perms,
+    const FFITraitObject* values, const Fr* d_beta, const Fr* d_gamma, const Fr* d_delta, const Fr* d_omega,
+    Fr* d_deltaomega, uint64_t n, uint64_t offset, uint64_t batch, hipStream_t stream)
 {
     H2_CHECK_ARG(batch > 0, "permutation product needs at least one column");
-    RustError e = permutation_product_columns(den, num, perms, values, d_beta, d_gamma, d_delta, d_omega, d_deltaomega,
-        n, offset, batch, true, stream);
+    RustError e = split_powers(lo, hi, d_omega, n, stream);
     if (e.code)
         return e;
-    return permutation_product_finish(den, num, n, stream);
+    e = permutation_product_columns(den, num, perms, values, d_beta, d_gamma, d_delta, lo, hi, d_deltaomega, n,
+        offset, batch, true, stream);
+    if (e.code)
+        return e;
+    return permutation_product_finish(den, num, n, inv_ws, stream);
 }
 
 } // namespace
@@ -380,15 +512,16 @@ RustError _halo2_grand_product_device_inputs(const FFITraitObject* d_inout, cons
 size_t _halo2_permutation_product_max_len(size_t length, uint64_t free_bytes)
 {
     for (uint64_t l = length; l > 0; l >>= 1)
-        if (4 * 32 * l + 4096 < free_bytes)
+        if (_halo2_permutation_product_workspace_size(l) < free_bytes)
             return l;
     return 0;
 }
 
 uint64_t _halo2_permutation_product_workspace_size(uint64_t poly_length)
 {
-    // denominators, numerators, and one staged (permutation, value) column pair
-    return 4 * align_up(32 * poly_length, 256) + 256;
+    // denominators, numerators, one staged (permutation, value) column pair, omega tables,
+    // group totals of the inversion
+    return 4 * align_up(32 * poly_length, 256) + split_bytes(poly_length) + batch_invert_ws_bytes(poly_length) + 512;
 }
 
 RustError _halo2_permutation_product(const FFITraitObject* modified_values, const FFITraitObject* permutations,
@@ -404,10 +537,18 @@ RustError _halo2_permutation_product(const FFITraitObject* modified_values, cons
     Fr* num = (Fr*)ws.take(32 * length);
     Fr* perm = (Fr*)ws.take(32 * length);
     Fr* val = (Fr*)ws.take(32 * length);
+    Fr* lo = (Fr*)ws.take(32ull << kSplit);
+    Fr* hi = (Fr*)ws.take(32 * split_hi(length));
+    void* inv_ws = ws.take(batch_invert_ws_bytes(length));
     if (ws.overflow)
         return fail((int)hipErrorInvalidValue, "_halo2_permutation_product: workspace of %lu bytes is too small",
             (unsigned long)scratch_bytes);
     H2_CHECK_ARG(batch > 0, "_halo2_permutation_product: batch must be >= 1");
+    {
+        RustError e = split_powers(lo, hi, (const Fr*)omega_device, length, stream);
+        if (e.code)
+            return e;
+    }
     // the running accumulator is the caller's chunk buffer itself (permutation_commit.cu:84)
     H2_TRY(hipMemcpyAsync(den, modified_values->ptr, 32 * length, hipMemcpyHostToDevice, stream));
     for (size_t c = 0; c < batch; ++c) {
@@ -418,11 +559,11 @@ RustError _halo2_permutation_product(const FFITraitObject* modified_values, cons
         FFITraitObject p { (uint64_t*)perm }, v { (uint64_t*)val };
         // one column at a time through the device routine (offset already applied)
         RustError e = permutation_product_columns(den, num, &p, &v, (const Fr*)beta_device, (const Fr*)gamma_device,
-            (const Fr*)delta_device, (const Fr*)omega_device, (Fr*)deltaomega_device, length, 0, 1, c == 0, stream);
+            (const Fr*)delta_device, lo, hi, (Fr*)deltaomega_device, length, 0, 1, c == 0, stream);
         if (e.code)
             return e;
     }
-    RustError e = permutation_product_finish(den, num, length, stream);
+    RustError e = permutation_product_finish(den, num, length, inv_ws, stream);
     if (e.code)
         return e;
     H2_TRY(hipMemcpyAsync(modified_values->ptr, den, 32 * length, hipMemcpyDeviceToHost, stream));
@@ -432,7 +573,7 @@ RustError _halo2_permutation_product(const FFITraitObject* modified_values, cons
 
 uint64_t _halo2_permutation_product_device_inputs_workspace_size(uint64_t poly_length)
 {
-    return align_up(32 * poly_length, 256) + 256;
+    return align_up(32 * poly_length, 256) + split_bytes(poly_length) + batch_invert_ws_bytes(poly_length) + 512;
 }
 
 RustError _halo2_permutation_product_device_inputs(const FFITraitObject* modified_values_device,
@@ -444,11 +585,15 @@ RustError _halo2_permutation_product_device_inputs(const FFITraitObject* modifie
         return ok();
     Workspace ws(scratch, scratch_bytes);
     Fr* num = (Fr*)ws.take(32 * length);
+    Fr* lo = (Fr*)ws.take(32ull << kSplit);
+    Fr* hi = (Fr*)ws.take(32 * split_hi(length));
+    void* inv_ws = ws.take(batch_invert_ws_bytes(length));
     if (ws.overflow)
         return fail((int)hipErrorInvalidValue,
             "_halo2_permutation_product_device_inputs: workspace of %lu bytes is too small",
             (unsigned long)scratch_bytes);
-    return permutation_product_device((Fr*)modified_values_device->ptr, num, permutations_device, values_device,
+    return permutation_product_device((Fr*)modified_values_device->ptr, num, lo, hi, inv_ws, permutations_device,
+        values_device,
         (const Fr*)beta_device, (const Fr*)gamma_device, (const Fr*)delta_device, (const Fr*)omega_device,
         (Fr*)deltaomega_device, length, offset, batch, (hipStream_t)stream);
 }
@@ -458,19 +603,24 @@ RustError _halo2_permutation_product_device_inputs(const FFITraitObject* modifie
 size_t _halo2_commit_product_max_len(size_t length, uint64_t free_bytes)
 {
     for (uint64_t l = length; l > 0; l >>= 1)
-        if (5 * 32 * l < free_bytes)
+        if (_halo2_commit_product_workspace_size(l) < free_bytes)
             return l;
     return 0;
 }
 
-uint64_t _halo2_commit_product_workspace_size(uint64_t poly_length) { return 5 * align_up(32 * poly_length, 256) + 256; }
+uint64_t _halo2_commit_product_workspace_size(uint64_t poly_length)
+{
+    return 5 * align_up(32 * poly_length, 256) + batch_invert_ws_bytes(poly_length) + 512;
+}
 
+// `inv_ws` holds batch_invert_ws_bytes(n) bytes, or is null (device-input entry point:
+// no workspace argument, single-kernel inversion)
 static RustError commit_product_device(Fr* lp, const Fr* pi, const Fr* pt, const Fr* ci, const Fr* ct, const Fr* beta,
-    const Fr* gamma, uint64_t n, hipStream_t stream)
+    const Fr* gamma, uint64_t n, void* inv_ws, hipStream_t stream)
 {
     lookup_denominator_kernel<<<ew_grid(n), kEwBlock, 0, stream>>>(lp, pi, pt, beta, gamma, n);
     H2_LAUNCH_CHECK();
-    RustError e = batch_invert<Fr>(lp, n, stream);
+    RustError e = batch_invert<Fr>(lp, n, stream, inv_ws, inv_ws ? batch_invert_ws_bytes(n) : 0);
     if (e.code)
         return e;
     lookup_numerator_kernel<<<ew_grid(n), kEwBlock, 0, stream>>>(lp, ci, ct, beta, gamma, n);
@@ -490,6 +640,7 @@ RustError _halo2_commit_product(const FFITraitObject* lookup_product, const FFIT
     Fr* d[5];
     for (int i = 0; i < 5; ++i)
         d[i] = (Fr*)ws.take(32 * length);
+    void* inv_ws = ws.take(batch_invert_ws_bytes(length));
     if (ws.overflow)
         return fail((int)hipErrorInvalidValue, "_halo2_commit_product: workspace of %lu bytes is too small",
             (unsigned long)scratch_bytes);
@@ -498,7 +649,7 @@ RustError _halo2_commit_product(const FFITraitObject* lookup_product, const FFIT
         H2_TRY(hipMemcpyAsync(d[i + 1], (const uint8_t*)src[i]->ptr + 32 * offset, 32 * length, hipMemcpyHostToDevice,
             stream));
     RustError e = commit_product_device(d[0], d[1], d[2], d[3], d[4], (const Fr*)beta_device, (const Fr*)gamma_device,
-        length, stream);
+        length, inv_ws, stream);
     if (e.code)
         return e;
     H2_TRY(hipMemcpyAsync((uint8_t*)lookup_product->ptr + 32 * offset, d[0], 32 * length, hipMemcpyDeviceToHost,
@@ -516,7 +667,7 @@ RustError _halo2_commit_product_device_inputs(const FFITraitObject* d_lookup_pro
         return ok();
     return commit_product_device((Fr*)d_lookup_product->ptr + offset, (const Fr*)d_permuted_input->ptr + offset,
         (const Fr*)d_permuted_table->ptr + offset, (const Fr*)d_compressed_input->ptr + offset,
-        (const Fr*)d_compressed_table->ptr + offset, (const Fr*)beta_device, (const Fr*)gamma_device, length,
+        (const Fr*)d_compressed_table->ptr + offset, (const Fr*)beta_device, (const Fr*)gamma_device, length, nullptr,
         (hipStream_t)stream);
 }
 

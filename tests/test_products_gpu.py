@@ -17,7 +17,7 @@ def _dev_scalar(x_limbs):
     return dv.to_device(np.asarray(x_limbs, dtype=np.uint64).reshape(1, 4))
 
 
-@pytest.mark.parametrize("n", [1, 7, 255, 2048, 2049, 65536 + 3, 1 << 20])
+@pytest.mark.parametrize("n", [1, 7, 255, 2048, 2049, 4096 + 129, 65536 + 3, 1 << 20, 1 << 24, (1 << 24) + 2049])
 def test_batch_invert_fr(lib, co, n):
     from halo2_amd import _lib, device as dv
 
