@@ -188,6 +188,220 @@ __global__ void __launch_bounds__(T) group_finish_kernel(uint8_t* __restrict__ d
     }
 }
 
+// ------------------------------- without a workspace: totals in spare bits
+//
+// The bare entry point has no workspace, so the group totals travel in the two
+// spare top bits of the elements themselves (inputs are canonical, < m < 2^254):
+// slot s (s < 8) of group j is the 256 bits held by the 128 elements
+// [jC + 128 s, jC + 128 (s + 1)) and carries the total of group 8j + s, then its
+// inverse. Readers mask the spare bits; every spare bit is written by one thread.
+// The middle step inverts the totals in blocks of kMidPer (one exponentiation per
+// block, the blocks in parallel); the finishing groups run host-last (group j
+// hosts 8j .. 8j + 7), so no group overwrites a slot another still has to read.
+constexpr unsigned kSlot = 128; // elements per slot, 2 bits each
+constexpr unsigned kMidThreads = 64;
+constexpr unsigned kMidPer = kMidThreads * E; // totals per middle block
+
+template <class P>
+__device__ __forceinline__ F29<P> load_masked(const uint8_t* p)
+{
+    const uint4* q = (const uint4*)p;
+    const uint4 a = q[0], b = q[1];
+    const uint32_t v[8] = { a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w & 0x3fffffffu };
+    return F29<P>::from32(v);
+}
+
+__device__ __forceinline__ uint32_t* spare_word(uint8_t* data, uint64_t elem)
+{
+    return (uint32_t*)(data + 32 * elem) + 7;
+}
+
+__device__ __forceinline__ uint64_t slot_base(uint64_t g) { return (g >> 3) * C + (g & 7) * kSlot; }
+
+// the 256-bit value in the slot at element `base`
+__device__ __forceinline__ void slot_get(uint8_t* data, uint64_t base, uint32_t (&w)[8])
+{
+#pragma unroll
+    for (unsigned q = 0; q < 8; ++q) {
+        uint32_t v = 0;
+#pragma unroll
+        for (unsigned r = 0; r < 16; ++r)
+            v |= (*spare_word(data, base + 16 * q + r) >> 30) << (2 * r);
+        w[q] = v;
+    }
+}
+
+// store w into the slot at element `base`: per 16 elements, all loads before the stores
+__device__ __forceinline__ void slot_put(uint8_t* data, uint64_t base, const uint32_t (&w)[8])
+{
+#pragma unroll
+    for (unsigned q = 0; q < 8; ++q) {
+        uint32_t old[16];
+#pragma unroll
+        for (unsigned r = 0; r < 16; ++r)
+            old[r] = *spare_word(data, base + 16 * q + r);
+#pragma unroll
+        for (unsigned r = 0; r < 16; ++r)
+            *spare_word(data, base + 16 * q + r) = (old[r] & 0x3fffffffu) | (((w[q] >> (2 * r)) & 3u) << 30);
+    }
+}
+
+// (1) group totals into the host slots
+template <class P>
+__global__ void __launch_bounds__(T) spare_total_kernel(uint8_t* __restrict__ data, uint64_t n)
+{
+    using F = F29<P>;
+    __shared__ F red[T];
+    __shared__ uint32_t word[8];
+    const unsigned tid = threadIdx.x;
+    const uint64_t base = (uint64_t)blockIdx.x * C;
+    F acc = F::one();
+#pragma unroll
+    for (unsigned k = 0; k < E; ++k) {
+        const uint64_t i = base + (uint64_t)k * T + tid;
+        if (i < n) {
+            const F a = load_masked<P>(data + 32 * i);
+            if (!a.is_zero())
+                acc = F::mul(acc, a);
+        }
+    }
+    red[tid] = acc;
+    for (unsigned d = T / 2; d > 0; d >>= 1) {
+        __syncthreads();
+        if (tid < d)
+            red[tid] = F::mul(red[tid], red[tid + d]);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t v[8];
+        F::reduce_full(red[0]).to32(v);
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            word[q] = v[q];
+    }
+    __syncthreads();
+    if (tid < kSlot) { // one element's two bits per thread
+        const uint32_t bits = (word[tid >> 4] >> (2 * (tid & 15))) & 3u;
+        uint32_t* sw = spare_word(data, slot_base(blockIdx.x) + tid);
+        *sw = (*sw & 0x3fffffffu) | (bits << 30);
+    }
+}
+
+// (2) block b inverts totals [b kMidPer, (b + 1) kMidPer): each becomes t^-1 * 2^-10
+// in the 2^261 scale, the single-group kernel's inv_total
+template <class P>
+__global__ void __launch_bounds__(kMidThreads) spare_inverse_kernel(uint8_t* __restrict__ data, uint64_t groups)
+{
+    using F = F29<P>;
+    __shared__ F pre[kMidThreads];
+    __shared__ F suf[kMidThreads];
+    __shared__ F inv_all;
+    const unsigned tid = threadIdx.x;
+    const uint64_t g0 = (uint64_t)blockIdx.x * kMidPer + (uint64_t)tid * E;
+    F part[E];
+    F acc = F::one();
+#pragma unroll
+    for (unsigned k = 0; k < E; ++k) {
+        part[k] = acc;
+        if (g0 + k < groups) {
+            uint32_t w[8];
+            slot_get(data, slot_base(g0 + k), w);
+            acc = F::mul(acc, F::from32(w)); // a total is never zero
+        }
+    }
+    pre[tid] = acc;
+    suf[tid] = acc;
+    for (unsigned d = 1; d < kMidThreads; d <<= 1) {
+        __syncthreads();
+        const F a = tid >= d ? pre[tid - d] : F::one();
+        const F b = tid + d < kMidThreads ? suf[tid + d] : F::one();
+        __syncthreads();
+        if (tid >= d)
+            pre[tid] = F::mul(a, pre[tid]);
+        if (tid + d < kMidThreads)
+            suf[tid] = F::mul(b, suf[tid]);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        F k251 = F::zero();
+        k251.l[8] = 1u << 19; // 2^251 = 2^(232 + 19)
+        inv_all = F::mul(F::pow_m_minus_2(pre[kMidThreads - 1]), k251);
+    }
+    __syncthreads();
+    F inv = inv_all;
+    if (tid > 0)
+        inv = F::mul(inv, pre[tid - 1]);
+    if (tid + 1 < kMidThreads)
+        inv = F::mul(inv, suf[tid + 1]);
+#pragma unroll
+    for (int k = (int)E - 1; k >= 0; --k) {
+        if (g0 + k < groups) {
+            uint32_t w[8], o[8];
+            slot_get(data, slot_base(g0 + k), w);
+            F::reduce_full(F::mul(inv, part[k])).to32(o);
+            inv = F::mul(inv, F::from32(w));
+            slot_put(data, slot_base(g0 + k), o);
+        }
+    }
+}
+
+// (3) groups [g0, g0 + gridDim.x): inverse total from the host slot, then as the
+// single-group kernel (outputs are canonical, so their spare bits end up zero)
+template <class P>
+__global__ void __launch_bounds__(T) spare_finish_kernel(uint8_t* __restrict__ data, uint64_t n, uint64_t g0)
+{
+    using F = F29<P>;
+    __shared__ F pre[T];
+    __shared__ F suf[T];
+    __shared__ uint32_t word[8];
+    const unsigned tid = threadIdx.x;
+    const uint64_t grp = g0 + blockIdx.x;
+    const uint64_t base = grp * C;
+    if (tid < 8)
+        word[tid] = 0;
+    __syncthreads();
+    if (tid < kSlot)
+        atomicOr(&word[tid >> 4], (*spare_word(data, slot_base(grp) + tid) >> 30) << (2 * (tid & 15)));
+    F part[E];
+    F acc = F::one();
+#pragma unroll
+    for (unsigned k = 0; k < E; ++k) {
+        const uint64_t i = base + (uint64_t)k * T + tid;
+        part[k] = acc;
+        if (i < n) {
+            const F a = load_masked<P>(data + 32 * i);
+            if (!a.is_zero())
+                acc = F::mul(acc, a);
+        }
+    }
+    pre[tid] = acc;
+    suf[tid] = acc;
+    block_scan_inclusive(pre, tid, false); // its first barrier also completes word[]
+    block_scan_inclusive(suf, tid, true);
+    uint32_t w[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        w[q] = word[q];
+    F inv = F::from32(w);
+    if (tid > 0)
+        inv = F::mul(inv, pre[tid - 1]);
+    if (tid + 1 < T)
+        inv = F::mul(inv, suf[tid + 1]);
+#pragma unroll
+    for (int k = (int)E - 1; k >= 0; --k) {
+        const uint64_t i = base + (uint64_t)k * T + tid;
+        if (i < n) {
+            const F a = load_masked<P>(data + 32 * i);
+            if (!a.is_zero()) {
+                F::reduce_full(F::mul(inv, part[k])).store32(data + 32 * i);
+                inv = F::mul(inv, a);
+            } else {
+                F::zero().store32(data + 32 * i); // clears spare bits a slot left on a zero
+            }
+        }
+    }
+}
+
 template <class F>
 struct Form29;
 template <>
@@ -206,7 +420,8 @@ uint64_t batch_invert_ws_bytes(uint64_t n)
     return groups > 1 ? align_up(32 * groups, 256) : 0;
 }
 
-// In place; with `ws` of batch_invert_ws_bytes(n) bytes the three-step path runs.
+// In place. With `ws` of batch_invert_ws_bytes(n) bytes the totals sit in ws;
+// without, they travel in the elements' spare bits.
 template <class F>
 RustError batch_invert(F* data, uint64_t n, hipStream_t stream, void* ws = nullptr, uint64_t ws_bytes = 0)
 {
@@ -215,9 +430,24 @@ RustError batch_invert(F* data, uint64_t n, hipStream_t stream, void* ws = nullp
         return ok();
     const uint64_t groups = (n + C - 1) / C;
     H2_CHECK_ARG(groups < (1ull << 31), "batch invert of %lu elements is too large", (unsigned long)n);
-    if (groups == 1 || ws == nullptr || ws_bytes < 32 * groups) {
-        batch_invert_kernel<P><<<(unsigned)groups, T, 0, stream>>>((uint8_t*)data, n);
+    if (groups == 1) {
+        batch_invert_kernel<P><<<1, T, 0, stream>>>((uint8_t*)data, n);
         H2_LAUNCH_CHECK();
+        return ok();
+    }
+    if (ws == nullptr || ws_bytes < 32 * groups) {
+        uint8_t* d = (uint8_t*)data;
+        spare_total_kernel<P><<<(unsigned)groups, T, 0, stream>>>(d, n);
+        H2_LAUNCH_CHECK();
+        spare_inverse_kernel<P><<<(unsigned)((groups + kMidPer - 1) / kMidPer), kMidThreads, 0, stream>>>(d, groups);
+        H2_LAUNCH_CHECK();
+        // hosts after the groups they host: [ceil(g/8), g), then [ceil(g/64), ceil(g/8)), ...
+        for (uint64_t hi = groups; hi > 0;) {
+            const uint64_t lo = hi > 1 ? (hi + 7) / 8 : 0;
+            spare_finish_kernel<P><<<(unsigned)(hi - lo), T, 0, stream>>>(d, n, lo);
+            H2_LAUNCH_CHECK();
+            hi = lo;
+        }
         return ok();
     }
     uint8_t* tot = (uint8_t*)ws;
