@@ -140,7 +140,8 @@ RustError _halo2_distribute_powers_zeta(void* d_a, const void* d_coset_powers, u
 /* ================================== products, scans and batch inversion */
 
 /* culib.rs:72-80 ; cuda/src/polynomial/operations.cu:16-42 — in place x -> 1/x, 0 -> 0;
- * field_type 0 = Fr, 1 = Fq */
+ * field_type 0 = Fr, 1 = Fq. Inputs are canonical (< m), as in the reference; during the
+ * call the two top bits of the elements carry intermediate values (the result is canonical). */
 RustError _halo2_batch_invert(void* d_data, uint32_t field_type, size_t length, void* stream);
 
 /* culib.rs:127 ; cuda/src/commitment/permute_expression_pair.cu:202-227 */
