@@ -23,9 +23,8 @@ using namespace h2;
 
 namespace {
 
-constexpr unsigned kThreads = 256;
-constexpr unsigned kPer = 8;
-constexpr unsigned kTile = kThreads * kPer; // 2048 keys, 64 KB of LDS
+constexpr unsigned kThreads = 256; // element-wise kernels
+constexpr unsigned kTile = 2048; // keys per sorted tile and per merge work-group, 64 KB of LDS
 
 struct Key {
     uint64_t w[4]; // canonical value, little-endian 64-bit words
@@ -120,7 +119,8 @@ __device__ __forceinline__ uint64_t merge_path(uint64_t na, uint64_t nb, uint64_
     return lo;
 }
 
-// Merges up to kPer outputs starting at diagonal `diag` of A|B held in LDS.
+// Merges up to P outputs starting at diagonal `diag` of A|B held in LDS.
+template <unsigned P>
 __device__ __forceinline__ void merge_run(const Key* A, uint32_t na, const Key* B, uint32_t nb, uint32_t diag,
     Key* out)
 {
@@ -129,7 +129,7 @@ __device__ __forceinline__ void merge_run(const Key* A, uint32_t na, const Key* 
     uint32_t i = i0, j = diag - i0;
     const uint32_t total = na + nb;
 #pragma unroll
-    for (unsigned k = 0; k < kPer; ++k) {
+    for (unsigned k = 0; k < P; ++k) {
         if (diag + k < total) {
             const bool takeA = j >= nb || (i < na && !key_lt(B[j], A[i]));
             out[k] = takeA ? A[i] : B[j];
@@ -141,94 +141,150 @@ __device__ __forceinline__ void merge_run(const Key* A, uint32_t na, const Key* 
     }
 }
 
-// Sorts each 2048-key tile of keys[y * stride + (0, U)) into out.
-__global__ void __launch_bounds__(kThreads) tile_sort_kernel(const Key* __restrict__ keys, Key* __restrict__ out,
-    uint64_t U, uint64_t stride)
+// Sorts each 2048-key tile of keys[y * stride + (0, U)) into out: eight keys
+// per thread sorted in registers, then eight merge rounds in LDS.
+constexpr unsigned kSortThreads = 256;
+constexpr unsigned kSortPer = kTile / kSortThreads;
+
+__global__ void __launch_bounds__(kSortThreads) tile_sort_kernel(const Key* __restrict__ keys,
+    Key* __restrict__ out, uint64_t U, uint64_t stride)
 {
     __shared__ Key sh[kTile];
     const Key* src = keys + blockIdx.y * stride;
     Key* dst = out + blockIdx.y * stride;
     const uint64_t base = (uint64_t)blockIdx.x * kTile;
     const uint32_t t = threadIdx.x;
-    for (uint32_t k = t; k < kTile; k += kThreads)
+    for (uint32_t k = t; k < kTile; k += kSortThreads)
         sh[k] = base + k < U ? key_load(src + base + k) : key_max();
     __syncthreads();
-    Key r[kPer];
+    Key r[kSortPer];
 #pragma unroll
-    for (unsigned k = 0; k < kPer; ++k)
-        r[k] = sh[t * kPer + k];
-    // odd-even transposition sort of 8 keys in registers
+    for (unsigned k = 0; k < kSortPer; ++k)
+        r[k] = sh[t * kSortPer + k];
+    // odd-even transposition sort of the thread's keys in registers
 #pragma unroll
-    for (unsigned phase = 0; phase < kPer; ++phase) {
+    for (unsigned phase = 0; phase < kSortPer; ++phase) {
 #pragma unroll
-        for (unsigned k = phase & 1; k + 1 < kPer; k += 2)
+        for (unsigned k = phase & 1; k + 1 < kSortPer; k += 2)
             cmp_swap(r[k], r[k + 1]);
     }
     __syncthreads();
 #pragma unroll
-    for (unsigned k = 0; k < kPer; ++k)
-        sh[t * kPer + k] = r[k];
+    for (unsigned k = 0; k < kSortPer; ++k)
+        sh[t * kSortPer + k] = r[k];
     __syncthreads();
-    for (uint32_t w = kPer; w < kTile; w <<= 1) {
-        const uint32_t d = t * kPer;
+    for (uint32_t w = kSortPer; w < kTile; w <<= 1) {
+        const uint32_t d = t * kSortPer;
         const uint32_t p = d / (2 * w);
         const Key* A = sh + p * 2 * w;
-        merge_run(A, w, A + w, w, d - p * 2 * w, r);
+        merge_run<kSortPer>(A, w, A + w, w, d - p * 2 * w, r);
         __syncthreads();
 #pragma unroll
-        for (unsigned k = 0; k < kPer; ++k)
+        for (unsigned k = 0; k < kSortPer; ++k)
             sh[d + k] = r[k];
         __syncthreads();
     }
-    for (uint32_t k = t; k < kTile; k += kThreads)
+    for (uint32_t k = t; k < kTile; k += kSortThreads)
         if (base + k < U)
             key_store(dst + base + k, sh[k]);
 }
 
-// One merge pass: runs of length w (sorted) merged pairwise into runs of 2w.
-__global__ void __launch_bounds__(kThreads) merge_pass_kernel(const Key* __restrict__ keys, Key* __restrict__ out,
-    uint64_t U, uint64_t stride, uint64_t w)
+// Geometry of output tile `tile` in a merge pass over runs of length w: the
+// pair of runs it falls in (A at `pair`, B after it) and its diagonal range.
+struct TileSpan {
+    uint64_t pair, na, nb, d0, d1;
+};
+
+__device__ __forceinline__ TileSpan tile_span(uint64_t tile, uint64_t U, uint64_t w)
+{
+    TileSpan t;
+    const uint64_t g0 = tile * kTile;
+    t.pair = g0 / (2 * w) * (2 * w);
+    const uint64_t rest = U - t.pair;
+    t.na = rest < w ? rest : w;
+    t.nb = rest > w ? (rest - w < w ? rest - w : w) : 0;
+    t.d0 = g0 - t.pair;
+    t.d1 = t.d0 + kTile < t.na + t.nb ? t.d0 + kTile : t.na + t.nb;
+    return t;
+}
+
+// Merge-path splits of one merge pass: for each key array y, output tile and
+// end (0 = first output, 1 = one past the last), the number of keys the tile
+// takes from run A, at splits[(y * tiles + tile) * 2 + end]. One wave per
+// split narrows the range with 64 probes per round (a 64-ary search), so a
+// run of 2^21 keys takes four rounds of dependent loads instead of 21.
+__global__ void __launch_bounds__(256) run_split_kernel(const Key* __restrict__ keys, uint64_t* __restrict__ splits,
+    uint64_t U, uint64_t stride, uint64_t w, uint32_t tiles)
+{
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t lane = threadIdx.x & 63u;
+    if (wave >= tiles * 4u) // uniform per wave
+        return;
+    const uint32_t y = wave / (tiles * 2u);
+    const uint32_t rem = wave % (tiles * 2u);
+    const TileSpan t = tile_span(rem >> 1, U, w);
+    const uint64_t d = (rem & 1u) ? t.d1 : t.d0;
+    const Key* A = keys + y * stride + t.pair;
+    const Key* B = A + t.na;
+    // first i in [lo, hi) with B[d - 1 - i] < A[i] (ties go to A), else hi
+    uint64_t lo = d > t.nb ? d - t.nb : 0;
+    uint64_t hi = d < t.na ? d : t.na;
+    while (lo < hi) {
+        const uint64_t step = (hi - lo + 63) >> 6;
+        const uint64_t i = lo + lane * step;
+        const bool p = i < hi && !key_lt(key_load(B + (d - 1 - i)), key_load(A + i));
+        const uint32_t c = (uint32_t)__popcll(__ballot(p)); // the predicate holds on a prefix of the probes
+        if (c == 0) {
+            hi = lo;
+        } else {
+            const uint64_t first_false = lo + c * step;
+            lo = lo + (c - 1) * step + 1;
+            if (first_false < hi)
+                hi = first_false;
+        }
+    }
+    if (lane == 0)
+        splits[wave] = lo;
+}
+
+// One merge pass: runs of length w (sorted) merged pairwise into runs of 2w,
+// 2048 outputs per work-group of 1024 threads (two each) between the splits
+// run_split_kernel found.
+constexpr unsigned kMergeThreads = 1024;
+constexpr unsigned kMergePer = kTile / kMergeThreads;
+
+__global__ void __launch_bounds__(kMergeThreads) merge_pass_kernel(const Key* __restrict__ keys,
+    Key* __restrict__ out, const uint64_t* __restrict__ splits, uint64_t U, uint64_t stride, uint64_t w)
 {
     __shared__ Key sh[kTile];
-    __shared__ uint64_t split[2];
     const Key* src = keys + blockIdx.y * stride;
     Key* dst = out + blockIdx.y * stride;
-    const uint64_t g0 = (uint64_t)blockIdx.x * kTile;
-    const uint64_t pair = g0 / (2 * w) * (2 * w);
-    const uint64_t rest = U - pair;
-    const uint64_t na = rest < w ? rest : w;
-    const uint64_t nb = rest > w ? (rest - w < w ? rest - w : w) : 0;
-    const Key* A = src + pair;
-    const Key* B = A + na;
-    const uint64_t d0 = g0 - pair;
-    const uint64_t d1 = d0 + kTile < na + nb ? d0 + kTile : na + nb;
-    const uint32_t t = threadIdx.x;
-    if (t < 2) {
-        const uint64_t diag = t == 0 ? d0 : d1;
-        split[t] = merge_path(na, nb, diag, [A](uint64_t k) { return key_load(A + k); },
-            [B](uint64_t k) { return key_load(B + k); });
-    }
-    __syncthreads();
-    const uint64_t a0 = split[0], a1 = split[1];
-    const uint64_t b0 = d0 - a0, b1 = d1 - a1;
+    const TileSpan ts = tile_span(blockIdx.x, U, w);
+    const Key* A = src + ts.pair;
+    const Key* B = A + ts.na;
+    const uint64_t* sp = splits + ((uint64_t)blockIdx.y * gridDim.x + blockIdx.x) * 2;
+    const uint64_t a0 = sp[0], a1 = sp[1];
+    const uint64_t b0 = ts.d0 - a0, b1 = ts.d1 - a1;
     const uint32_t ta = (uint32_t)(a1 - a0), tb = (uint32_t)(b1 - b0);
-    for (uint32_t k = t; k < ta + tb; k += kThreads)
+    const uint32_t t = threadIdx.x;
+    for (uint32_t k = t; k < ta + tb; k += kMergeThreads)
         sh[k] = k < ta ? key_load(A + a0 + k) : key_load(B + b0 + (k - ta));
     __syncthreads();
-    Key r[kPer];
+    Key r[kMergePer];
 #pragma unroll
-    for (unsigned k = 0; k < kPer; ++k)
+    for (unsigned k = 0; k < kMergePer; ++k)
         r[k] = key_max();
-    const uint32_t d = t * kPer;
+    const uint32_t d = t * kMergePer;
     const uint32_t total = ta + tb;
-    merge_run(sh, ta, sh + ta, tb, d < total ? d : total, r);
+    merge_run<kMergePer>(sh, ta, sh + ta, tb, d < total ? d : total, r);
     __syncthreads();
 #pragma unroll
-    for (unsigned k = 0; k < kPer; ++k)
+    for (unsigned k = 0; k < kMergePer; ++k)
         if (d + k < total)
             sh[d + k] = r[k];
     __syncthreads();
-    for (uint32_t k = t; k < ta + tb; k += kThreads)
+    const uint64_t g0 = (uint64_t)blockIdx.x * kTile;
+    for (uint32_t k = t; k < ta + tb; k += kMergeThreads)
         key_store(dst + g0 + k, sh[k]);
 }
 
@@ -309,6 +365,7 @@ struct PermBufs {
     uint32_t* leftover;
     uint32_t* rpos;
     uint32_t* lpos;
+    uint64_t* splits; // merge-path splits of one pass: 2 arrays x tiles x 2
     void* scan;
 };
 
@@ -322,6 +379,7 @@ PermBufs perm_carve(WS& ws, uint64_t U)
     b.leftover = (uint32_t*)ws.take(4 * U);
     b.rpos = (uint32_t*)ws.take(4 * (U + 1));
     b.lpos = (uint32_t*)ws.take(4 * (U + 1));
+    b.splits = (uint64_t*)ws.take(8 * 4 * ((U + kTile - 1) / kTile));
     b.scan = ws.take(scan_u32_scratch(U));
     return b;
 }
@@ -375,11 +433,15 @@ RustError _halo2_permute_expression_pair(const void* d_compressed_input, const v
             (const Fr*)d_compressed_table, b.keys[0], U, U);
         H2_LAUNCH_CHECK();
         const unsigned tiles = blocks_for(U, kTile);
-        tile_sort_kernel<<<dim3(tiles, 2), kThreads, 0, stream>>>(b.keys[0], b.keys[1], U, U);
+        tile_sort_kernel<<<dim3(tiles, 2), kSortThreads, 0, stream>>>(b.keys[0], b.keys[1], U, U);
         H2_LAUNCH_CHECK();
         int cur = 1;
         for (uint64_t w = kTile; w < U; w <<= 1) {
-            merge_pass_kernel<<<dim3(tiles, 2), kThreads, 0, stream>>>(b.keys[cur], b.keys[cur ^ 1], U, U, w);
+            run_split_kernel<<<blocks_for(4ull * tiles * 64, 256), 256, 0, stream>>>(b.keys[cur], b.splits, U, U, w,
+                tiles);
+            H2_LAUNCH_CHECK();
+            merge_pass_kernel<<<dim3(tiles, 2), kMergeThreads, 0, stream>>>(b.keys[cur], b.keys[cur ^ 1], b.splits, U, U,
+                w);
             H2_LAUNCH_CHECK();
             cur ^= 1;
         }
