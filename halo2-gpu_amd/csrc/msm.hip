@@ -31,6 +31,7 @@
 // The result is written as (x, y, 1) or the identity (0, 1, 0) in Montgomery
 // form, the layout of the reference's `cukernel_to_affine` output.
 #include <memory>
+#include <vector>
 
 #include "common.h"
 #include "curve29.h"
@@ -55,6 +56,15 @@ constexpr uint32_t RED_L1 = 16; // buckets per first-level reduction segment
 constexpr uint32_t TREE = 256; // threads (tree leaves) per work-group at later levels
 constexpr uint32_t TREE_K = 8; // pairs folded sequentially into each leaf
 constexpr uint32_t MAX_RED_LEVELS = 8;
+constexpr uint32_t MAX_JOBS = 64; // MSMs in one batched pipeline
+
+// One MSM of a batch: its points occupy the virtual range [voff, voff + n)
+// (voff a multiple of R1_TILE, so a round-1 tile belongs to one job) and the
+// round-1 tiles [tile0, tile0 + tiles).
+struct BatchJob {
+    uint64_t voff, n;
+    uint32_t tile0, tiles;
+};
 
 // 128-byte slot holding one XYZZ point (four 32-byte coordinates)
 struct PointSlot {
@@ -62,13 +72,16 @@ struct PointSlot {
 };
 
 struct MsmPlan {
-    uint64_t n;
+    uint64_t n; // points (for a batch: the virtual range of all jobs)
+    uint64_t nbase; // bases converted to the 2^261 scale (a batch shares one base array)
+    uint32_t J; // MSMs in the pipeline (1 unless batched); each has its own W bucket sets
     uint32_t c, W, B;
     uint32_t Wb; // bucket windows: W, or 1 when the windows share buckets (precomputed bases)
     uint64_t table_n; // precomputed bases: row stride of the table (0 = plain bases)
     uint64_t nb; // Wb * B buckets
     uint32_t FB, CB; // round-2 (fine) and round-1 (coarse) bucket bits, FB + CB = c - 1
-    uint32_t NC; // coarse bins = Wb << CB
+    uint32_t NCj; // coarse bins per MSM = (W, or 1 for shared buckets) << CB
+    uint32_t NC; // coarse bins = J * NCj
     uint32_t ntiles; // round-1 tiles
     uint64_t max_chunks; // upper bound on round-2 chunks
     uint32_t S; // sorted entries per accumulation thread
@@ -102,21 +115,24 @@ uint32_t plan_width(uint64_t n, bool shared)
     return (255 + w0 - 1) / w0;
 }
 
-// n points; table_n > 0 selects precomputed bases whose table was built for
-// table_n points (the window width follows the table).
-MsmPlan make_plan(uint64_t n, uint64_t table_n = 0)
+// n points (virtual range), window width c, J MSMs; table_n > 0 selects
+// precomputed bases (one MSM, shared buckets).
+MsmPlan plan_core(uint64_t n, uint32_t c, uint32_t J, uint64_t table_n)
 {
     MsmPlan p {};
     p.n = n;
+    p.nbase = n;
+    p.J = J;
     p.table_n = table_n;
-    p.c = plan_width(table_n ? table_n : n, table_n != 0);
+    p.c = c;
     p.W = (255 + p.c - 1) / p.c;
-    p.Wb = table_n ? 1u : p.W;
+    p.Wb = table_n ? 1u : J * p.W;
     p.B = 1u << (p.c - 1);
     p.nb = (uint64_t)p.Wb * p.B;
     p.FB = p.c - 1 < MAX_FB ? p.c - 1 : MAX_FB;
     p.CB = p.c - 1 - p.FB;
-    p.NC = p.Wb << p.CB;
+    p.NCj = (table_n ? 1u : p.W) << p.CB;
+    p.NC = J * p.NCj;
     p.ntiles = (uint32_t)((n + R1_TILE - 1) / R1_TILE);
     const uint64_t work = n * p.W;
     p.max_chunks = work / R2_CHUNK + p.NC;
@@ -146,8 +162,40 @@ MsmPlan make_plan(uint64_t n, uint64_t table_n = 0)
     return p;
 }
 
+// n points; table_n > 0 selects precomputed bases whose table was built for
+// table_n points (the window width follows the table).
+MsmPlan make_plan(uint64_t n, uint64_t table_n = 0)
+{
+    return plan_core(n, plan_width(table_n ? table_n : n, table_n != 0), 1, table_n);
+}
+
+uint64_t align_tile(uint64_t n) { return (n + R1_TILE - 1) / R1_TILE * R1_TILE; }
+
+// J MSMs over one shared base array: one width for all (from the mean job
+// size), each job's points at a tile-aligned virtual offset.
+MsmPlan make_batch_plan(const uint64_t* lengths, uint32_t J)
+{
+    uint64_t nv = 0, nmax = 0;
+    for (uint32_t j = 0; j < J; ++j) {
+        nv += align_tile(lengths[j]);
+        if (lengths[j] > nmax)
+            nmax = lengths[j];
+    }
+    const uint64_t mean = nv / J ? nv / J : 1;
+    uint32_t c = plan_width(mean, false);
+    if (c > 16) // J * W * 2^(c-1) buckets of 128 B: 4 GiB at most for 64 jobs
+        c = 16;
+    MsmPlan p = plan_core(nv ? nv : 1, c, J, 0);
+    p.n = nv;
+    p.ntiles = (uint32_t)(nv / R1_TILE);
+    p.nbase = nmax;
+    return p;
+}
+
 struct MsmBuffers {
-    uint8_t* pts; // n affine bases in the 2^261 scale (64 B each)
+    uint8_t* pts; // nbase affine bases in the 2^261 scale (64 B each)
+    BatchJob* jobs; // J (batches only)
+    uint32_t* tile_job; // job of each round-1 tile (batches only)
     uint32_t* canon; // n x 8 limbs
     uint32_t* h1; // NC * ntiles
     uint32_t* h1off; // NC * ntiles + 1
@@ -163,13 +211,13 @@ struct MsmBuffers {
     PointSlot* bsum; // nb
     PointSlot* red[4]; // (T, S) ping-pong: W * Mlev[0] each
     void* scan_scratch;
-    uint64_t* out; // 12 limbs
+    uint64_t* out; // 12 limbs per MSM
 };
 
 template <class WS>
 void carve(WS& ws, const MsmPlan& p, MsmBuffers* b)
 {
-    const uint64_t nh1 = (uint64_t)p.NC * p.ntiles;
+    const uint64_t nh1 = (uint64_t)p.NCj * p.ntiles; // every tile counts the bins of its own MSM
     const uint64_t nh2 = p.max_chunks << p.FB;
     uint64_t scan_n = nh1 + 1;
     if (nh2 + 1 > scan_n)
@@ -177,7 +225,9 @@ void carve(WS& ws, const MsmPlan& p, MsmBuffers* b)
     const uint64_t nred = (uint64_t)p.Wb * p.Mlev[0];
     void* v[24];
     int k = 0;
-    v[k++] = ws.take(p.table_n ? 0 : 64 * p.n);
+    v[k++] = ws.take(p.table_n ? 0 : 64 * p.nbase);
+    v[k++] = ws.take(p.J > 1 ? sizeof(BatchJob) * p.J : 0);
+    v[k++] = ws.take(p.J > 1 ? 4ull * p.ntiles : 0);
     v[k++] = ws.take(32 * p.n);
     v[k++] = ws.take(4 * nh1);
     v[k++] = ws.take(4 * (nh1 + 1));
@@ -194,10 +244,12 @@ void carve(WS& ws, const MsmPlan& p, MsmBuffers* b)
     for (int i = 0; i < 4; ++i)
         v[k++] = ws.take(128 * nred);
     v[k++] = ws.take(scan_u32_scratch(scan_n));
-    v[k++] = ws.take(96);
+    v[k++] = ws.take(96ull * p.J);
     if (b) {
         k = 0;
         b->pts = (uint8_t*)v[k++];
+        b->jobs = (BatchJob*)v[k++];
+        b->tile_job = (uint32_t*)v[k++];
         b->canon = (uint32_t*)v[k++];
         b->h1 = (uint32_t*)v[k++];
         b->h1off = (uint32_t*)v[k++];
@@ -287,7 +339,31 @@ struct SortParams {
     uint64_t n;
     uint32_t c, W, FB, CB, NC, ntiles;
     uint64_t table_n; // > 0: windows share the bins and entry w of point i indexes row w of the table
+    uint32_t NCj; // coarse bins per MSM
+    const BatchJob* jobs; // null for a single MSM
+    const uint32_t* tile_job;
 };
+
+// The MSM a round-1 tile belongs to: its virtual point range and tile range.
+// The round-1 histogram h1 is job-major, then bin-major, then tile: job j's
+// bin b of its tile t sits at NCj * tile0 + b * tiles + (t - tile0).
+struct TileRange {
+    uint64_t v0, vend;
+    uint32_t tile0, tiles;
+};
+
+__device__ __forceinline__ TileRange tile_range(const SortParams& sp, uint32_t tile)
+{
+    if (!sp.jobs)
+        return TileRange { 0, sp.n, 0, sp.ntiles };
+    const BatchJob jb = sp.jobs[sp.tile_job[tile]];
+    return TileRange { jb.voff, jb.voff + jb.n, jb.tile0, jb.tiles };
+}
+
+__device__ __forceinline__ uint64_t h1_index(const SortParams& sp, const TileRange& r, uint32_t b, uint32_t tile)
+{
+    return (uint64_t)sp.NCj * r.tile0 + (uint64_t)b * r.tiles + (tile - r.tile0);
+}
 
 // Round 1, pass 1: per tile of R1_TILE points, LDS histogram over the coarse
 // bins (window, top CB bucket bits); stored bin-major so the linear scan
@@ -296,13 +372,14 @@ __global__ void __launch_bounds__(R1_THREADS) r1_count_kernel(const uint32_t* __
     uint32_t* __restrict__ h1)
 {
     extern __shared__ uint32_t hist[];
-    for (uint32_t i = threadIdx.x; i < sp.NC; i += R1_THREADS)
+    const TileRange tr = tile_range(sp, blockIdx.x);
+    for (uint32_t i = threadIdx.x; i < sp.NCj; i += R1_THREADS)
         hist[i] = 0;
     __syncthreads();
     const uint64_t p0 = (uint64_t)blockIdx.x * R1_TILE;
     for (uint32_t k = 0; k < R1_PER; ++k) {
         const uint64_t i = p0 + (uint64_t)k * R1_THREADS + threadIdx.x;
-        if (i >= sp.n)
+        if (i >= tr.vend)
             break;
         uint32_t s[8];
 #pragma unroll
@@ -318,8 +395,8 @@ __global__ void __launch_bounds__(R1_THREADS) r1_count_kernel(const uint32_t* __
         }
     }
     __syncthreads();
-    for (uint32_t b = threadIdx.x; b < sp.NC; b += R1_THREADS)
-        h1[(uint64_t)b * sp.ntiles + blockIdx.x] = hist[b];
+    for (uint32_t b = threadIdx.x; b < sp.NCj; b += R1_THREADS)
+        h1[h1_index(sp, tr, b, blockIdx.x)] = hist[b];
 }
 
 // Round 1, pass 2: the same tile places its entries at the scanned offsets.
@@ -339,13 +416,14 @@ __global__ void __launch_bounds__(R1_THREADS) r1_scatter_kernel(const uint32_t* 
 {
     extern __shared__ uint32_t sm[];
     const uint32_t BW = 1u << sp.CB;
-    uint32_t* cursor = sm; // NC: this tile's start in every (window, bin) run
-    uint32_t* base = cursor + ((sp.NC + 1) & ~1u); // BW + 1: per-window bin counts, then their exclusive scan
+    uint32_t* cursor = sm; // NCj: this tile's start in every (window, bin) run of its MSM
+    uint32_t* base = cursor + ((sp.NCj + 1) & ~1u); // BW + 1: per-window bin counts, then their exclusive scan
     uint64_t* stage = (uint64_t*)(base + ((BW + 2) & ~1u)); // R1_TILE entries, bin-ordered
     uint16_t* sbin = (uint16_t*)(stage + R1_TILE); // the bin of each staged entry
     __shared__ uint32_t wsum[R1_THREADS / 64];
-    for (uint32_t b = threadIdx.x; b < sp.NC; b += R1_THREADS)
-        cursor[b] = h1off[(uint64_t)b * sp.ntiles + blockIdx.x];
+    const TileRange tr = tile_range(sp, blockIdx.x);
+    for (uint32_t b = threadIdx.x; b < sp.NCj; b += R1_THREADS)
+        cursor[b] = h1off[h1_index(sp, tr, b, blockIdx.x)];
     const uint32_t fmask = (1u << sp.FB) - 1u;
     const uint64_t p0 = (uint64_t)blockIdx.x * R1_TILE;
     uint32_t s[R1_PER][8];
@@ -356,7 +434,7 @@ __global__ void __launch_bounds__(R1_THREADS) r1_scatter_kernel(const uint32_t* 
         carry[k] = 0;
 #pragma unroll
         for (int q = 0; q < 8; ++q)
-            s[k][q] = i < sp.n ? canon[8 * i + q] : 0u; // a zero scalar has only zero digits
+            s[k][q] = i < tr.vend ? canon[8 * i + q] : 0u; // a zero scalar has only zero digits
     }
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (uint32_t w = 0; w < sp.W; ++w) {
@@ -373,7 +451,7 @@ __global__ void __launch_bounds__(R1_THREADS) r1_scatter_kernel(const uint32_t* 
             if (d != 0) {
                 const uint32_t bi = (uint32_t)(d < 0 ? -d : d) - 1u;
                 bin[k] = bi >> sp.FB;
-                const uint32_t idx = sp.table_n ? (uint32_t)(w * sp.table_n + i) : (uint32_t)i;
+                const uint32_t idx = sp.table_n ? (uint32_t)(w * sp.table_n + i) : (uint32_t)(i - tr.v0);
                 ent[k] = ((uint64_t)(bi & fmask) << 32) | idx | (d < 0 ? 0x80000000u : 0u);
                 rank[k] = atomicAdd(&base[bin[k]], 1u);
             }
@@ -434,26 +512,37 @@ __global__ void __launch_bounds__(R1_THREADS) r1_scatter_kernel(const uint32_t* 
     }
 }
 
+// Start of global coarse bin g (g <= NC) in the round-1 order: job-major,
+// then bin, then tile (bin g = j * NCj + b of job j).
+__device__ __forceinline__ uint32_t bin_lo(const uint32_t* __restrict__ h1off, const SortParams& sp, uint32_t g)
+{
+    if (!sp.jobs)
+        return h1off[(uint64_t)g * sp.ntiles];
+    if (g >= sp.NC)
+        return h1off[(uint64_t)sp.NCj * sp.ntiles];
+    const BatchJob jb = sp.jobs[g / sp.NCj];
+    return h1off[(uint64_t)sp.NCj * jb.tile0 + (uint64_t)(g % sp.NCj) * jb.tiles];
+}
+
 // One work-group: bin boundaries from the round-1 scan, and the number of
 // round-2 chunks per bin (at least one, so every bucket offset gets written).
+// Thread t owns the contiguous bins [t * per, (t + 1) * per).
 __global__ void __launch_bounds__(1024) bin_info_kernel(const uint32_t* __restrict__ h1off, SortParams sp,
     uint32_t* __restrict__ bin_start, uint32_t* __restrict__ chunk_base, uint32_t* __restrict__ off0_end)
 {
     __shared__ uint32_t s[1024];
-    constexpr uint32_t PER = 4; // NC <= 4096
-    uint32_t k[PER];
+    const uint32_t per = (sp.NC + 1023) / 1024;
+    const uint32_t g0 = threadIdx.x * per;
+    auto chunks = [&](uint32_t g, uint32_t& lo) -> uint32_t {
+        lo = bin_lo(h1off, sp, g);
+        const uint32_t sz = bin_lo(h1off, sp, g + 1) - lo;
+        return sz == 0 ? 1u : (sz + R2_CHUNK - 1) / R2_CHUNK;
+    };
     uint32_t sum = 0;
-    for (uint32_t q = 0; q < PER; ++q) {
-        const uint32_t b = threadIdx.x * PER + q;
-        k[q] = 0;
-        if (b < sp.NC) {
-            const uint32_t lo = h1off[(uint64_t)b * sp.ntiles];
-            const uint32_t hi = h1off[(uint64_t)(b + 1) * sp.ntiles];
-            bin_start[b] = lo;
-            const uint32_t sz = hi - lo;
-            k[q] = sz == 0 ? 1u : (sz + R2_CHUNK - 1) / R2_CHUNK;
-        }
-        sum += k[q];
+    for (uint32_t q = 0; q < per; ++q) {
+        uint32_t lo;
+        if (g0 + q < sp.NC)
+            sum += chunks(g0 + q, lo);
     }
     s[threadIdx.x] = sum;
     for (uint32_t d = 1; d < 1024; d <<= 1) {
@@ -464,14 +553,18 @@ __global__ void __launch_bounds__(1024) bin_info_kernel(const uint32_t* __restri
     }
     __syncthreads();
     uint32_t run = threadIdx.x ? s[threadIdx.x - 1] : 0u;
-    for (uint32_t q = 0; q < PER; ++q) {
-        const uint32_t b = threadIdx.x * PER + q;
-        if (b < sp.NC)
-            chunk_base[b] = run;
-        run += k[q];
+    for (uint32_t q = 0; q < per; ++q) {
+        const uint32_t g = g0 + q;
+        if (g < sp.NC) {
+            uint32_t lo;
+            const uint32_t k = chunks(g, lo);
+            bin_start[g] = lo;
+            chunk_base[g] = run;
+            run += k;
+        }
     }
     if (threadIdx.x == 0) {
-        const uint32_t total = h1off[(uint64_t)sp.NC * sp.ntiles];
+        const uint32_t total = bin_lo(h1off, sp, sp.NC);
         bin_start[sp.NC] = total;
         chunk_base[sp.NC] = s[1023];
         *off0_end = total;
@@ -807,11 +900,15 @@ __global__ void __launch_bounds__(TREE) reduce_tree_kernel(const PointSlot* __re
     }
 }
 
+// One thread per MSM: Horner over its W window sums (win[j * W + w]).
 __global__ void __launch_bounds__(64) finalize_kernel(const PointSlot* __restrict__ win, uint32_t W, uint32_t c,
-    uint64_t* __restrict__ out)
+    uint32_t J, uint64_t* __restrict__ out)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0)
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= J)
         return;
+    win += (uint64_t)j * W;
+    out += 12ull * j;
     G1X29 acc = G1X29::load(win + (W - 1));
     for (int w = (int)W - 2; w >= 0; --w) {
         for (uint32_t k = 0; k < c; ++k)
@@ -879,29 +976,34 @@ unsigned blocks_for(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t)
 
 uint32_t ilog2(uint32_t x) { return 31 - __builtin_clz(x); }
 
-// Full device pipeline; the 96-byte result lands in b.out (device).
+// Full device pipeline; the 96-byte results land in b.out (device). A batch
+// (p.J > 1) has its canonical scalars and job tables in place already and
+// passes d_scalars = null.
 RustError msm_device(const MsmPlan& p, const MsmBuffers& b, const Fr* d_scalars, const uint8_t* d_points,
     hipStream_t stream)
 {
     const unsigned T = 256;
     ProfScope prof_all("msm_total", stream);
     std::unique_ptr<ProfScope> prof_sort(new ProfScope("msm_sort", stream));
-    to_canonical_kernel<<<blocks_for(p.n, T), T, 0, stream>>>(d_scalars, b.canon, p.n);
-    H2_LAUNCH_CHECK();
+    if (d_scalars) {
+        to_canonical_kernel<<<blocks_for(p.n, T), T, 0, stream>>>(d_scalars, b.canon, p.n);
+        H2_LAUNCH_CHECK();
+    }
     const uint8_t* pts = d_points; // a precomputed table is already in the 2^261 scale
     if (!p.table_n) {
-        to_scale261_kernel<<<blocks_for(p.n, T), T, 0, stream>>>(d_points, b.pts, p.n);
+        to_scale261_kernel<<<blocks_for(p.nbase, T), T, 0, stream>>>(d_points, b.pts, p.nbase);
         H2_LAUNCH_CHECK();
         pts = b.pts;
     }
-    SortParams sp { p.n, p.c, p.W, p.FB, p.CB, p.NC, p.ntiles, p.table_n };
-    const uint64_t nh1 = (uint64_t)p.NC * p.ntiles;
-    r1_count_kernel<<<p.ntiles, R1_THREADS, 4 * p.NC, stream>>>(b.canon, sp, b.h1);
+    SortParams sp { p.n, p.c, p.W, p.FB, p.CB, p.NC, p.ntiles, p.table_n, p.NCj, p.J > 1 ? b.jobs : nullptr,
+        p.J > 1 ? b.tile_job : nullptr };
+    const uint64_t nh1 = (uint64_t)p.NCj * p.ntiles; // every tile counts the bins of its own MSM
+    r1_count_kernel<<<p.ntiles, R1_THREADS, 4 * p.NCj, stream>>>(b.canon, sp, b.h1);
     H2_LAUNCH_CHECK();
     RustError e = scan_u32_exclusive(b.h1, b.h1off, nh1, b.scan_scratch, stream);
     if (e.code)
         return e;
-    r1_scatter_kernel<<<p.ntiles, R1_THREADS, r1_scatter_lds(p.NC, p.CB), stream>>>(b.canon, sp, b.h1off, b.kv1);
+    r1_scatter_kernel<<<p.ntiles, R1_THREADS, r1_scatter_lds(p.NCj, p.CB), stream>>>(b.canon, sp, b.h1off, b.kv1);
     H2_LAUNCH_CHECK();
     bin_info_kernel<<<1, 1024, 0, stream>>>(b.h1off, sp, b.bin_start, b.chunk_base, b.off0 + p.nb);
     H2_LAUNCH_CHECK();
@@ -941,7 +1043,7 @@ RustError msm_device(const MsmPlan& p, const MsmBuffers& b, const Fr* d_scalars,
         H2_LAUNCH_CHECK();
         cur ^= 1;
     }
-    finalize_kernel<<<1, 64, 0, stream>>>(b.red[2 * cur], p.Wb, p.c, b.out);
+    finalize_kernel<<<blocks_for(p.J, 64), 64, 0, stream>>>(b.red[2 * cur], p.Wb / p.J, p.c, p.J, b.out);
     H2_LAUNCH_CHECK();
     return ok();
 }
@@ -1105,6 +1207,76 @@ RustError h2amd_multiexp_precomputed(const void* d_scalar, const void* d_table, 
     if (e.code)
         return e;
     H2_TRY(hipMemcpyAsync(o, b.out, 96, hipMemcpyDeviceToHost, stream));
+    H2_TRY(hipStreamSynchronize(stream));
+    return ok();
+}
+
+// ------------------------------------------ batched MSMs (no reference counterpart)
+
+uint64_t h2amd_multiexp_batch_workspace_size(const uint64_t* lengths, uint32_t count)
+{
+    if (count == 0 || count > MAX_JOBS)
+        return 0;
+    const MsmPlan p = make_batch_plan(lengths, count);
+    SizerAdapter sa;
+    carve(sa, p, nullptr);
+    return sa.s.bytes + 4096;
+}
+
+RustError h2amd_multiexp_batch(const void* const* d_scalars, const uint64_t* lengths, uint32_t count,
+    const void* d_bases, void* h_out, void* scratch, uint64_t scratch_bytes, void* stream_)
+{
+    hipStream_t stream = (hipStream_t)stream_;
+    H2_CHECK_ARG(count <= MAX_JOBS, "h2amd_multiexp_batch: %u MSMs exceed %u per batch", count, MAX_JOBS);
+    if (count == 0)
+        return ok();
+    uint64_t* o = (uint64_t*)h_out;
+    uint64_t nv = 0;
+    for (uint32_t j = 0; j < count; ++j) {
+        H2_CHECK_ARG(lengths[j] < (1ull << 31), "h2amd_multiexp_batch: length %lu of MSM %u exceeds 2^31",
+            (unsigned long)lengths[j], j);
+        nv += align_tile(lengths[j]);
+    }
+    if (nv == 0) { // every MSM is empty: identities (0, 1, 0)
+        memset(o, 0, 96ull * count);
+        const Fq one = Fq::one();
+        for (uint32_t j = 0; j < count; ++j)
+            memcpy(o + 12 * j + 4, one.v, 32);
+        return ok();
+    }
+    const MsmPlan p = make_batch_plan(lengths, count);
+    H2_CHECK_ARG(p.n * p.W < (1ull << 32), "h2amd_multiexp_batch: %lu padded points too many for 32-bit offsets",
+        (unsigned long)p.n);
+    Workspace ws(scratch, scratch_bytes);
+    MsmBuffers b {};
+    carve(ws, p, &b);
+    if (ws.overflow)
+        return fail((int)hipErrorInvalidValue, "h2amd_multiexp_batch: workspace of %lu bytes is too small",
+            (unsigned long)scratch_bytes);
+    std::vector<BatchJob> jobs(count);
+    std::vector<uint32_t> tile_job(p.ntiles);
+    uint64_t voff = 0;
+    for (uint32_t j = 0; j < count; ++j) {
+        const uint64_t span = align_tile(lengths[j]);
+        jobs[j] = BatchJob { voff, lengths[j], (uint32_t)(voff / R1_TILE), (uint32_t)(span / R1_TILE) };
+        for (uint32_t t = 0; t < jobs[j].tiles; ++t)
+            tile_job[jobs[j].tile0 + t] = j;
+        voff += span;
+    }
+    H2_TRY(hipMemcpyAsync(b.jobs, jobs.data(), sizeof(BatchJob) * count, hipMemcpyHostToDevice, stream));
+    H2_TRY(hipMemcpyAsync(b.tile_job, tile_job.data(), 4ull * p.ntiles, hipMemcpyHostToDevice, stream));
+    for (uint32_t j = 0; j < count; ++j) {
+        if (lengths[j] == 0)
+            continue;
+        to_canonical_kernel<<<blocks_for(lengths[j], 256), 256, 0, stream>>>((const Fr*)d_scalars[j],
+            b.canon + 8 * jobs[j].voff, lengths[j]);
+        H2_LAUNCH_CHECK();
+    }
+    RustError e = msm_device(p, b, nullptr, (const uint8_t*)d_bases, stream);
+    if (e.code)
+        return e;
+    H2_TRY(hipMemcpyAsync(o, b.out, 96ull * count, hipMemcpyDeviceToHost, stream));
+    // also keeps the host job tables alive until their copies are done
     H2_TRY(hipStreamSynchronize(stream));
     return ok();
 }

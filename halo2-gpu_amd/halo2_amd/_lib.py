@@ -140,6 +140,8 @@ SIGNATURES = {
     "h2amd_msm_precompute": (RustError, [P, P, U64, P, U64, P]),
     "h2amd_multiexp_precomputed_workspace_size": (U64, [U64, U64]),
     "h2amd_multiexp_precomputed": (RustError, [P, P, U64, OBJP, U64, P, U64, P]),
+    "h2amd_multiexp_batch_workspace_size": (U64, [P, U32]),
+    "h2amd_multiexp_batch": (RustError, [P, P, U32, P, P, P, U64, P]),
     "h2amd_random_fr": (RustError, [P, U64, U64, U64, P]),
     "h2amd_g1_mul_generator_workspace_size": (U64, [U64]),
     "h2amd_g1_mul_generator": (RustError, [P, P, U64, P, U64, P]),

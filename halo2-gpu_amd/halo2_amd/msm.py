@@ -120,6 +120,24 @@ def multiexp_precomputed(d_scalars: torch.Tensor, table: torch.Tensor, table_n: 
     return out
 
 
+def multiexp_batch_device(d_scalars: list[torch.Tensor], d_bases: torch.Tensor,
+                          lengths: list[int] | None = None) -> np.ndarray:
+    """`h2amd_multiexp_batch`: MSM j of the first lengths[j] (default: all) scalars of
+    d_scalars[j] against the shared bases, all in one pipeline; (count, 12) uint64."""
+    lib = _lib.load()
+    count = len(d_scalars)
+    if lengths is None:
+        lengths = [x.numel() // 4 for x in d_scalars]
+    lens = np.ascontiguousarray(lengths, dtype=np.uint64)
+    ptrs = (ctypes.c_void_p * max(1, count))(*[dv.ptr(x) for x in d_scalars])
+    ws = int(lib.h2amd_multiexp_batch_workspace_size(lens.ctypes.data, count))
+    scratch = dv.empty_bytes(ws)
+    out = np.zeros((max(1, count), 12), dtype=np.uint64)
+    _lib.check(lib.h2amd_multiexp_batch(ptrs, lens.ctypes.data, count, dv.ptr(d_bases), out.ctypes.data,
+                                        dv.ptr(scratch), ws, dv.stream()), "h2amd_multiexp_batch")
+    return out[:count]
+
+
 def g1_mul_generator(d_scalars: torch.Tensor, n: int) -> torch.Tensor:
     """d_out[i] = s_i * G (affine, 8 words per point) on the device."""
     lib = _lib.load()
@@ -214,6 +232,13 @@ class ParamsKZG:
         """commitment.rs:467-485: MSM of device Lagrange values against the resident
         Lagrange basis."""
         return self._commit_resident(d_values, "g_lagrange")
+
+    def commit_many_device(self, d_polys: list[torch.Tensor]) -> np.ndarray:
+        """KZG commitments of many device polynomials (any lengths <= n) against the
+        resident SRS in one batched MSM pipeline, at most 64 per launch sequence;
+        the same points commit_device gives one by one."""
+        outs = [multiexp_batch_device(d_polys[i:i + 64], self.g_device) for i in range(0, len(d_polys), 64)]
+        return np.concatenate(outs) if outs else np.zeros((0, 12), dtype=np.uint64)
 
     def commit_lagrange(self, h_values: np.ndarray) -> np.ndarray:
         """commitment.rs:456-466: host Lagrange values, resident Lagrange basis."""

@@ -397,6 +397,18 @@ uint64_t h2amd_multiexp_precomputed_workspace_size(uint64_t length, uint64_t tab
 RustError h2amd_multiexp_precomputed(const void* d_scalar, const void* d_table, uint64_t table_n,
     const FFITraitObject* h_output, uint64_t length, void* scratch, uint64_t scratch_bytes, void* stream);
 
+/* ======================= batched MSMs (no reference counterpart)
+ * Up to 64 MSMs of uneven lengths over one shared device base array, in one sort /
+ * accumulate / reduce pipeline: MSM j = sum_{i < lengths[j]} s_j[i] * bases[i] (the
+ * KZG commitments of many polynomials against one SRS, which the reference runs as
+ * one `_halo2_multiexp_device_scalars_device_bases` call each, multiexp.rs:215-256).
+ * d_scalars: `count` device pointers (host array); d_bases: at least max(lengths)
+ * affine bases; h_out: count x 96 bytes, each equal bit for bit to the single-MSM
+ * result. Synchronises. */
+uint64_t h2amd_multiexp_batch_workspace_size(const uint64_t* lengths, uint32_t count);
+RustError h2amd_multiexp_batch(const void* const* d_scalars, const uint64_t* lengths, uint32_t count,
+    const void* d_bases, void* h_out, void* scratch, uint64_t scratch_bytes, void* stream);
+
 /* ======================= kernel timing (no reference counterpart)
  * When enabled, launchers bracket their hot kernels ("ntt_pass", "msm_bucket_level0",
  * ...) with HIP events on the launch stream; reading syncs the recorded events. */

@@ -281,3 +281,91 @@ def test_kzg_fixed_base_commit(lib, co):
     assert (params.commit_device(dv.to_device(poly)) == plain.commit_device(dv.to_device(poly))).all()
     assert (params.commit(poly[:100]) == plain.commit(poly[:100])).all()
     assert (params.commit_lagrange(poly) == plain.commit_lagrange(poly)).all()
+
+
+def _batch_case(co, lengths, seed, nmax):
+    """Device bases s_i G (i < nmax) and per-job scalars: expected results from the
+    closed form (sum_i c_i s_i) G on the oracle."""
+    from halo2_amd import device as dv, msm
+
+    ds = msm.random_fr(nmax, seed)
+    pts = msm.g1_mul_generator(ds, nmax)
+    hs = dv.to_host(ds)
+    scal, exp = [], []
+    for j, n in enumerate(lengths):
+        c = rand_fr(co, n, seed * 100 + j) if n else np.zeros((0, 4), dtype=np.uint64)
+        if j % 5 == 3 and n:  # a dense job: every scalar equal (one bucket per window)
+            c = np.tile(_dense_constant(), (n, 1))
+        if j % 7 == 6 and n:  # zero scalars
+            c[: n // 2] = 0
+        scal.append(c)
+        exp.append(co.g1_mul(co.g1_generator(), co.fr_inner_product(c, hs[:n])) if n else None)
+    return pts, scal, exp
+
+
+def _identity():
+    from halo2_amd import fields as F
+
+    return np.concatenate([np.zeros(4, np.uint64), F.mont_limbs(1, F.P_MOD), np.zeros(4, np.uint64)])
+
+
+def test_msm_batch_uneven(lib, co):
+    """MSMs of uneven lengths (empty, tile edges, 2^16 + a few, 2^18) in one batched
+    pipeline: each equals its closed form and the single-MSM result."""
+    import torch
+
+    from halo2_amd import device as dv, msm
+
+    lengths = [0, 1, 3, 4095, 4096, 4097, 12345, 1 << 16, (1 << 16) + 5, 1, 1 << 18, 77, 0, 5000]
+    nmax = max(lengths)
+    print(f"msm batch {len(lengths)} jobs, max {nmax}", flush=True)
+    pts, scal, exp = _batch_case(co, lengths, 81, nmax)
+    d = [dv.to_device(c) if len(c) else torch.zeros(4, dtype=torch.int64, device=pts.device) for c in scal]
+    got = msm.multiexp_batch_device(d, pts, lengths)
+    for j, n in enumerate(lengths):
+        if n == 0:
+            assert (got[j] == _identity()).all(), f"job {j} empty"
+            continue
+        assert (got[j] == exp[j]).all(), f"job {j} n={n}"
+        if n in (4097, 1 << 16):
+            assert (got[j] == msm.multiexp_device_scalars_device_bases(d[j], pts, n)).all()
+
+
+def test_msm_batch_many_jobs_and_limits(lib, co):
+    """64 small jobs (the most one batch takes) against closed forms; 65 is refused and
+    an all-empty batch gives identities."""
+    import torch
+
+    from halo2_amd import _lib, device as dv, msm
+
+    rng = np.random.default_rng(5)
+    lengths = [int(x) for x in rng.integers(1, 3000, 64)]
+    pts, scal, exp = _batch_case(co, lengths, 82, max(lengths))
+    d = [dv.to_device(c) for c in scal]
+    got = msm.multiexp_batch_device(d, pts)
+    for j in range(64):
+        assert (got[j] == exp[j]).all(), f"job {j}"
+    with pytest.raises(_lib.Halo2GpuError):
+        msm.multiexp_batch_device(d + d[:1], pts)
+    z = torch.zeros(4, dtype=torch.int64, device=pts.device)
+    got = msm.multiexp_batch_device([z, z], pts, [0, 0])
+    assert (got == _identity()).all()
+
+
+def test_kzg_commit_many(lib, co):
+    """ParamsKZG.commit_many_device: polynomials of several lengths committed in one
+    batch equal commit_device one by one and f(tau) G."""
+    from halo2_amd import device as dv, fields as F, msm
+
+    k = 14
+    n = 1 << k
+    tau = 0xABCDEF0123456789 % F.R_MOD
+    params = msm.ParamsKZG(k, tau, fixed_base=False)
+    pows = co.fr_powers(F.mont_limbs(tau), n)
+    lens = [n, 100, n // 2 + 1, 4096, 1]
+    polys = [rand_fr(co, m, 90 + i) for i, m in enumerate(lens)]
+    got = params.commit_many_device([dv.to_device(p) for p in polys])
+    for j, p in enumerate(polys):
+        exp = co.g1_mul(co.g1_generator(), co.fr_inner_product(p, pows[: len(p)]))
+        assert (got[j] == exp).all(), f"poly {j}"
+        assert (params.commit_device(dv.to_device(p)) == exp).all()
