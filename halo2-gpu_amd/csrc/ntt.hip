@@ -7,6 +7,8 @@
 //   cosetFFT      out = DFT_omega(pad(in)_i * zeta^(i mod 3)) of size 2^extend_log_n
 //   icosetFFT     out_i = divisor * DFT_omega(pad(in))_i * zeta^-(i mod 3)
 //   CosetFFT_Part out = DFT_omega(in_i * c^i), c = divisor (g_coset * ext_omega^p)
+#include <vector>
+
 #include "ntt_core.h"
 
 using namespace h2;
@@ -310,6 +312,83 @@ int32_t _halo2_get_fft_split_radix(uint32_t ntt_type, uint32_t log_n, uint32_t e
             return (int32_t)s;
     }
     return -1;
+}
+
+// ----------------------------------------- uneven batches (no reference counterpart)
+
+uint64_t h2amd_fft_batch_to_device_workspace_size(uint32_t ntt_type, uint32_t count, const uint32_t* log_n,
+    const uint32_t* extend_log_n)
+{
+    WorkspaceSizer s;
+    for (uint32_t i = 0; i < count; ++i) {
+        const uint32_t la = active_log(ntt_type, log_n[i], extend_log_n[i]);
+        if (la < 1 || la > NTT_MAX_LOG || log_n[i] > la)
+            return 0;
+        ntt_tables_size(s, la, ntt_type == 6);
+        if (ntt_plan(la).P == 3)
+            s.take(32ull << la);
+    }
+    s.take(ntt_batch_pass_bytes((int)count));
+    return s.bytes + 4096;
+}
+
+RustError h2amd_fft_batch_to_device(uint32_t ntt_type, uint32_t count, const uint32_t* log_n,
+    const uint32_t* extend_log_n, const void* const* d_in, void* const* d_out, const void* const* d_omega,
+    const FFITraitObject* divisors, void* scratch, uint64_t scratch_bytes, void* stream_)
+{
+    const char* who = "h2amd_fft_batch_to_device";
+    hipStream_t stream = (hipStream_t)stream_;
+    if (ntt_type == 4)
+        return fail(2, "icosetFFT not supported now");
+    if (ntt_type == 5)
+        return fail(-1, "iFFT_cosetFFT not supported");
+    if (!kind_ok(ntt_type))
+        return fail(-6, "NTT type %u not supported", ntt_type);
+    if (count == 0)
+        return ok();
+    NttTwist tw;
+    NttPost post;
+    kind_params(ntt_type, tw, post);
+    Workspace ws(scratch, scratch_bytes);
+    std::vector<NttTables> tabs(count);
+    std::vector<uint32_t> owner(count), las(count);
+    std::vector<Fr> divs(count);
+    std::vector<NttBatchJob> jobs(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        if (log_n[i] == 0 || extend_log_n[i] == 0)
+            return fail((int)hipErrorInvalidValue, "%s: log_n and extend_log_n must be >= 1 (job %u)", who, i);
+        const uint32_t la = active_log(ntt_type, log_n[i], extend_log_n[i]);
+        RustError e = check_log(who, la);
+        if (e.code)
+            return e;
+        if (log_n[i] > la)
+            return fail((int)hipErrorInvalidValue, "%s: job %u: log_n %u exceeds the transform size 2^%u", who, i,
+                log_n[i], la);
+        las[i] = la;
+        divs[i] = host_fr(divisors[i].ptr);
+        // jobs of one size, omega (and coset factor) share their tables
+        owner[i] = i;
+        for (uint32_t k = 0; k < i; ++k) {
+            const bool same_c = tw != TWIST_POWERS || !memcmp(divs[k].v, divs[i].v, 32);
+            if (las[k] == la && d_omega[k] == d_omega[i] && same_c) {
+                owner[i] = owner[k];
+                break;
+            }
+        }
+        if (owner[i] == i) {
+            e = ntt_tables_build(ws, tabs[i], la, (const Fr*)d_omega[i], tw == TWIST_POWERS, divs[i], stream);
+            if (e.code)
+                return e;
+        }
+        Fr* scr = ntt_plan(la).P == 3 ? (Fr*)ws.take(32ull << la) : nullptr;
+        jobs[i] = NttBatchJob { &tabs[owner[i]], la, 1ull << log_n[i], (const Fr*)d_in[i], (Fr*)d_out[i], scr,
+            divs[i] };
+    }
+    void* d_pass = ws.take(ntt_batch_pass_bytes((int)count));
+    if (ws.overflow)
+        return fail((int)hipErrorInvalidValue, "%s: workspace of %lu bytes is too small", who,
+            (unsigned long)scratch_bytes);
+    return ntt_run_batch(jobs.data(), (int)count, tw, post, d_pass, stream);
 }
 
 } // extern "C"

@@ -76,4 +76,28 @@ RustError ntt_run(const NttTables& t, uint32_t log_n, const NttJob* jobs, int co
 
 bool ntt_needs_scratch(uint32_t log_n, bool aliased);
 
+// One transform of an uneven batch: its own size and tables (shared between
+// jobs of one size), `in_len` valid inputs (the rest read as zero), in != out
+// when the transform takes two passes, and a scratch buffer of 2^log_n
+// elements when it takes three (log_n > 24).
+struct NttBatchJob {
+    const NttTables* tables;
+    uint32_t log_n;
+    uint64_t in_len;
+    const Fr* in;
+    Fr* out;
+    Fr* scratch;
+    Fr scale; // POST_SCALE / POST_SCALE_ZETA_INV factor
+};
+
+// Device bytes of the per-pass job tables of a batch of `count` transforms.
+uint64_t ntt_batch_pass_bytes(int count);
+
+// Runs every job's pass p in one launch (p = 0, 1, 2): work-groups of all
+// transforms side by side, so small transforms run concurrently with large
+// ones. `d_pass` holds ntt_batch_pass_bytes(count) device bytes. Synchronises
+// once, after uploading the job tables.
+RustError ntt_run_batch(const NttBatchJob* jobs, int count, NttTwist twist, NttPost post, void* d_pass,
+    hipStream_t stream);
+
 } // namespace h2

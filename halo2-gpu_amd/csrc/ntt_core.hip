@@ -5,6 +5,8 @@
 // field29.h: elements stay in the boundary's 2^256 Montgomery scale and every
 // constant the pass multiplies by (twiddles, coset powers, zeta, the iFFT
 // scale) is held in the 2^261 scale.
+#include <vector>
+
 #include "field29.h"
 #include "ntt_core.h"
 #include "profile.h"
@@ -234,21 +236,19 @@ __device__ __forceinline__ void lds_put(uint32_t* lds, uint32_t RG, uint32_t i, 
 // every value is < (1.25 + 2s) m < 25.25m < 2^259 with limbs < 2^29 + 2^30,
 // within mul()'s operand limits; the write multiplies by the inter-pass
 // twiddle (or the scale), or reduces fully in the last pass.
-__global__ void __launch_bounds__(1024) ntt_pass_kernel(PassBatch pb, PassParams pp)
+// Work-group `b` (of nwg) of one pass of one transform.
+__device__ __forceinline__ void ntt_pass_tile(
+    const Fr* __restrict__ in, Fr* __restrict__ out, const PassParams& pp, uint32_t b, uint32_t* lds)
 {
-    extern __shared__ uint32_t lds[];
     const uint32_t log_r = pp.log_r, log_g = pp.log_g, log_l = pp.log_l;
     const uint32_t R = 1u << log_r;
     const uint32_t G = 1u << log_g;
     const uint32_t RG = R << log_g;
     const uint32_t T = blockDim.x;
     const uint32_t nwg = (uint32_t)((1ull << pp.log_n) >> (log_r + log_g));
-    uint32_t b = blockIdx.x;
     if (pp.xcd_remap) // blocks are dealt to the 8 XCDs round robin: keep neighbouring columns on one L2
         b = (b & 7u) * (nwg >> 3) + (b >> 3);
     const uint32_t j0 = b << log_g;
-    const Fr* __restrict__ in = pb.in[blockIdx.y];
-    Fr* __restrict__ out = pb.out[blockIdx.y];
     const uint64_t stride = (1ull << pp.log_n) >> log_r;
 
     for (uint32_t f = threadIdx.x; f < RG; f += T) {
@@ -373,6 +373,37 @@ __global__ void __launch_bounds__(1024) ntt_pass_kernel(PassBatch pb, PassParams
     }
 }
 
+__global__ void __launch_bounds__(1024) ntt_pass_kernel(PassBatch pb, PassParams pp)
+{
+    extern __shared__ uint32_t lds[];
+    ntt_pass_tile(pb.in[blockIdx.y], pb.out[blockIdx.y], pp, blockIdx.x, lds);
+}
+
+// One pass of many transforms of different sizes: work-groups
+// [block0, block0 + nwg) of the launch belong to the transform.
+struct BatchPass {
+    const Fr* in;
+    Fr* out;
+    uint32_t block0, nwg;
+    PassParams pp;
+};
+
+__global__ void __launch_bounds__(1024) ntt_pass_batch_kernel(const BatchPass* __restrict__ jobs, uint32_t njobs)
+{
+    extern __shared__ uint32_t lds[];
+    uint32_t lo = 0, hi = njobs; // last job with block0 <= blockIdx.x (uniform)
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (jobs[mid].block0 <= blockIdx.x)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const BatchPass& jb = jobs[lo];
+    const PassParams pp = jb.pp;
+    ntt_pass_tile(jb.in, jb.out, pp, blockIdx.x - jb.block0, lds);
+}
+
 static bool g_attr_set = false;
 
 RustError ntt_run(const NttTables& t, uint32_t log_n, const NttJob* jobs, int count, uint64_t in_len,
@@ -459,6 +490,100 @@ RustError ntt_run(const NttTables& t, uint32_t log_n, const NttJob* jobs, int co
             }
             H2_LAUNCH_CHECK();
         }
+    }
+    return ok();
+}
+
+// ---------------------------------------------------------- uneven batches
+
+uint64_t ntt_batch_pass_bytes(int count) { return 3ull * (uint64_t)count * sizeof(BatchPass); }
+
+RustError ntt_run_batch(const NttBatchJob* jobs, int count, NttTwist twist, NttPost post, void* d_pass,
+    hipStream_t stream)
+{
+    if (count <= 0)
+        return ok();
+    static bool attr = false;
+    if (!attr) {
+        H2_TRY(hipFuncSetAttribute((const void*)ntt_pass_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+            36 << NTT_MAX_LOG_R));
+        attr = true;
+    }
+    std::vector<BatchPass> host(3ull * count);
+    uint32_t nlaunch[3] = { 0, 0, 0 }, grid[3] = { 0, 0, 0 }, rgmax[3] = { 0, 0, 0 };
+    for (int i = 0; i < count; ++i) {
+        const NttBatchJob& jb = jobs[i];
+        H2_CHECK_ARG(jb.log_n >= 1 && jb.log_n <= NTT_MAX_LOG, "NTT batch: log_n=%u out of range [1, %u]", jb.log_n,
+            NTT_MAX_LOG);
+        NttPlan p = ntt_plan(jb.log_n);
+        if (p.P > 1) // whole 4096-element tiles: a batch fills the chip with many transforms, not wide ones
+            for (int q = 0; q < p.P; ++q) {
+                const uint32_t g = NTT_MAX_LOG_R - p.log_r[q];
+                p.log_g[q] = g < jb.log_n - p.log_r[q] ? g : jb.log_n - p.log_r[q];
+            }
+        H2_CHECK_ARG(p.P < 3 || jb.scratch, "NTT batch: 2^%u needs a scratch buffer", jb.log_n);
+        H2_CHECK_ARG(p.P != 2 || (const void*)jb.in != (const void*)jb.out, "NTT batch: in-place 2^%u transform",
+            jb.log_n);
+        uint32_t scale261[9] = { 0 };
+        if (post != POST_NONE)
+            host_to261(jb.scale, scale261);
+        for (int pass = 0; pass < p.P; ++pass) {
+            BatchPass bp {};
+            if (p.P == 1) {
+                bp.in = jb.in;
+                bp.out = jb.out;
+            } else if (p.P == 2) {
+                bp.in = pass == 0 ? jb.in : jb.out;
+                bp.out = jb.out;
+            } else {
+                bp.in = pass == 0 ? jb.in : (pass == 1 ? jb.scratch : jb.out);
+                bp.out = pass == 0 ? jb.scratch : jb.out;
+            }
+            PassParams& pp = bp.pp;
+            pp.log_n = jb.log_n;
+            pp.log_r = p.log_r[pass];
+            pp.log_g = p.log_g[pass];
+            uint32_t log_l = 0;
+            for (int q = 0; q < pass; ++q)
+                log_l += p.log_r[q];
+            pp.log_l = log_l;
+            pp.in_len = pass == 0 ? jb.in_len : (1ull << jb.log_n);
+            pp.tw_lo = jb.tables->tw_lo;
+            pp.tw_hi = jb.tables->tw_hi;
+            pp.h = jb.tables->h;
+            pp.twr = jb.tables->twr[pass];
+            pp.c_lo = jb.tables->c_lo;
+            pp.c_hi = jb.tables->c_hi;
+            pp.twist = pass == 0 ? (int)twist : TWIST_NONE;
+            pp.last = pass == p.P - 1;
+            pp.post = pp.last ? (int)post : POST_NONE;
+            for (int q = 0; q < 9; ++q)
+                pp.scale[q] = scale261[q];
+            pp.xcd_remap = 0;
+            const uint32_t rg = 1u << (pp.log_r + pp.log_g);
+            bp.nwg = (uint32_t)((1ull << jb.log_n) >> (pp.log_r + pp.log_g));
+            bp.block0 = grid[pass];
+            grid[pass] += bp.nwg;
+            if (rg > rgmax[pass])
+                rgmax[pass] = rg;
+            host[(uint64_t)pass * count + nlaunch[pass]++] = bp;
+        }
+    }
+    H2_TRY(hipMemcpyAsync(d_pass, host.data(), sizeof(BatchPass) * host.size(), hipMemcpyHostToDevice, stream));
+    H2_TRY(hipStreamSynchronize(stream)); // the host table goes away on return
+    const BatchPass* dp = (const BatchPass*)d_pass;
+    for (int pass = 0; pass < 3; ++pass) {
+        if (!nlaunch[pass])
+            continue;
+        uint32_t T = rgmax[pass] / 2;
+        if (T > 1024)
+            T = 1024;
+        if (T < 64)
+            T = 64;
+        ProfScope prof("ntt_pass_batch", stream);
+        ntt_pass_batch_kernel<<<grid[pass], T, 36u * rgmax[pass], stream>>>(dp + (uint64_t)pass * count,
+            nlaunch[pass]);
+        H2_LAUNCH_CHECK();
     }
     return ok();
 }

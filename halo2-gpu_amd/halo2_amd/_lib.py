@@ -74,6 +74,8 @@ SIGNATURES = {
     "_halo2_fft_many_to_device_workspace_size": (U64, [U32, U32, U32]),
     "_halo2_fft_many": (RustError, [U32, U32, U32, U32, OBJP, OBJP, P, OBJP, P, U64, P]),
     "_halo2_fft_many_to_device": (RustError, [U32, U32, U32, U32, OBJP, OBJP, P, OBJP, BOOL, P, U64, P]),
+    "h2amd_fft_batch_to_device_workspace_size": (U64, [U32, U32, P, P]),
+    "h2amd_fft_batch_to_device": (RustError, [U32, U32, P, P, P, P, P, OBJP, P, U64, P]),
     "_halo2_fft_many_check_memory": (BOOL, [U32, U32, U32]),
     "_halo2_get_fft_split_radix": (I32, [U32, U32, U32, U64]),
     "_halo2_extended_from_lagrange_vec_device": (RustError, [P, P, U32, U64, P]),

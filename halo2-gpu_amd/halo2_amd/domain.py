@@ -121,3 +121,13 @@ class EvaluationDomain:
         out = dv.empty_u64(4 * self.n * len(parts))
         ntt.extended_from_lagrange_vec_device(out, parts, self.n)
         return out
+
+
+def coeff_to_extended_batch_device(domains: list[EvaluationDomain], polys: list[torch.Tensor]) -> list[torch.Tensor]:
+    """coeff_to_extended (domain.rs:597-613) of polynomials from domains of different sizes,
+    every NTT pass of all of them in one launch: poly i (n_i device coefficients) becomes
+    its domain's extended evaluations, as domains[i].coeff_to_extended_many_device would
+    give one by one."""
+    assert len(domains) == len(polys)
+    return ntt.fft_batch_to_device(ntt.COSET_FFT, polys, [d.k for d in domains], [d.extended_k for d in domains],
+                                   [d.extended_omega for d in domains], [1] * len(domains))

@@ -13,7 +13,7 @@ import torch
 
 from . import _lib
 from . import device as dv
-from .fields import mont_bytes, mont_limbs
+from .fields import R_MOD, mont_bytes, mont_limbs
 
 FFT, IFFT, COSET_FFT, ICOSET_FFT, IFFT_COSET_FFT, COSET_FFT_PART = 1, 2, 3, 4, 5, 6
 
@@ -110,6 +110,38 @@ def fft_many_to_device(ntt_type: int, inputs, log_n: int, extend_log_n: int, ome
                                         ctypes.byref(div_obj), on_dev, dv.ptr(scratch), ws, dv.stream())
     _lib.check(err, "_halo2_fft_many_to_device")
     del keep
+    return outputs
+
+
+def fft_batch_to_device(ntt_type: int, inputs: list[torch.Tensor], log_ns: list[int], extend_log_ns: list[int],
+                        omegas: list[int], divisors: list[int],
+                        outputs: list[torch.Tensor] | None = None) -> list[torch.Tensor]:
+    """`h2amd_fft_batch_to_device`: transforms of uneven sizes (each as
+    `_halo2_fft_many_to_device` would run it alone), every pass of all of them in one
+    launch. Device inputs and outputs; outputs are allocated when not given."""
+    lib = _lib.load()
+    count = len(inputs)
+    if count == 0:
+        return []
+    coset = ntt_type in (COSET_FFT, ICOSET_FFT)
+    if outputs is None:
+        outputs = [dv.empty_u64(4 << (e if coset else k)) for k, e in zip(log_ns, extend_log_ns)]
+    ln = np.ascontiguousarray(log_ns, dtype=np.uint32)
+    ex = np.ascontiguousarray(extend_log_ns, dtype=np.uint32)
+    d_omegas = {}
+    for w in omegas:  # one device word per distinct omega, so equal sizes share their tables
+        if w % R_MOD not in d_omegas:
+            d_omegas[w % R_MOD] = _scalar_dev(w)
+    om = (ctypes.c_void_p * count)(*[dv.ptr(d_omegas[w % R_MOD]) for w in omegas])
+    ins = (ctypes.c_void_p * count)(*[dv.ptr(x) for x in inputs])
+    outs = (ctypes.c_void_p * count)(*[dv.ptr(x) for x in outputs])
+    divs = [np.ascontiguousarray(mont_limbs(d)) for d in divisors]
+    div_objs = _lib.obj_array([d.ctypes.data for d in divs])
+    ws = int(lib.h2amd_fft_batch_to_device_workspace_size(ntt_type, count, ln.ctypes.data, ex.ctypes.data))
+    scratch = dv.empty_bytes(ws)
+    _lib.check(lib.h2amd_fft_batch_to_device(ntt_type, count, ln.ctypes.data, ex.ctypes.data, ins, outs, om,
+                                             div_objs, dv.ptr(scratch), ws, dv.stream()), "h2amd_fft_batch_to_device")
+    del divs
     return outputs
 
 

@@ -397,6 +397,19 @@ uint64_t h2amd_multiexp_precomputed_workspace_size(uint64_t length, uint64_t tab
 RustError h2amd_multiexp_precomputed(const void* d_scalar, const void* d_table, uint64_t table_n,
     const FFITraitObject* h_output, uint64_t length, void* scratch, uint64_t scratch_bytes, void* stream);
 
+/* ======================= uneven NTT batches (no reference counterpart)
+ * `count` transforms of one kind (FFT=1, iFFT=2, cosetFFT=3, CosetFFT_Part=6; as
+ * `_halo2_fft_many_to_device` with device input) but each with its own log_n /
+ * extend_log_n, omega (device word) and divisor (host word): every pass of all of them
+ * runs in one launch, so small transforms fill the chip beside large ones (the
+ * coeff_to_extended of all polynomials of a proof, which the reference runs per size).
+ * Outputs must not alias inputs; results equal the per-transform entry points. */
+uint64_t h2amd_fft_batch_to_device_workspace_size(uint32_t ntt_type, uint32_t count, const uint32_t* log_n,
+    const uint32_t* extend_log_n);
+RustError h2amd_fft_batch_to_device(uint32_t ntt_type, uint32_t count, const uint32_t* log_n,
+    const uint32_t* extend_log_n, const void* const* d_in, void* const* d_out, const void* const* d_omega,
+    const FFITraitObject* divisors, void* scratch, uint64_t scratch_bytes, void* stream);
+
 /* ======================= batched MSMs (no reference counterpart)
  * Up to 64 MSMs of uneven lengths over one shared device base array, in one sort /
  * accumulate / reduce pipeline: MSM j = sum_{i < lengths[j]} s_j[i] * bases[i] (the

@@ -329,3 +329,69 @@ def _libc_free(e):
 
     if e.message:
         _lib._libc.free(e.message)
+
+
+def test_fft_batch_uneven(lib, co, po):
+    """Uneven batches (one- and two-pass sizes side by side, 2^1 .. 2^18) of FFT, iFFT,
+    CosetFFT_Part and cosetFFT, each job against the oracle."""
+    from halo2_amd import device as dv, fields as F, ntt
+
+    ks = [1, 3, 5, 12, 13, 14, 16, 18, 14, 7]
+    print(f"fft batch sizes {ks}", flush=True)
+    polys = [rand_fr(co, 1 << k, 1300 + i) for i, k in enumerate(ks)]
+    d_in = [dv.to_device(p) for p in polys]
+    cs = [F.ZETA * pow(F.omega(k + 2), 3, F.R_MOD) % F.R_MOD for k in ks]
+    cs[8] = cs[5] + 1  # two jobs of one size with different coset factors
+    for kind in (1, 2, 6):
+        omegas = [F.omega(k) if kind != 2 else F.inv(F.omega(k)) for k in ks]
+        divs = {1: [1] * len(ks), 2: [F.inv(1 << k) for k in ks], 6: cs}[kind]
+        outs = ntt.fft_batch_to_device(kind, d_in, ks, ks, omegas, divs)
+        for i, k in enumerate(ks):
+            exp = _oracle_kind(co, po, kind, polys[i], k, c=divs[i])
+            assert (dv.to_host(outs[i]) == exp).all(), f"kind {kind} job {i} k={k}"
+        for p, d in zip(polys, d_in):
+            assert (dv.to_host(d) == p).all(), "inputs must not be modified"
+    pairs = [(3, 5), (10, 12), (14, 16), (16, 18), (1, 3)]
+    cpolys = [rand_fr(co, 1 << k, 1400 + k) for k, _ in pairs]
+    outs = ntt.fft_batch_to_device(3, [dv.to_device(p) for p in cpolys], [k for k, _ in pairs],
+                                   [e for _, e in pairs], [F.omega(e) for _, e in pairs], [1] * len(pairs))
+    for (k, e), p, o in zip(pairs, cpolys, outs):
+        assert (dv.to_host(o) == _oracle_kind(co, po, 3, p, k, ext_k=e)).all(), f"coset k={k} ext={e}"
+
+
+def test_fft_batch_three_pass_and_refusals(lib, co, po):
+    """A three-pass transform (2^25, scratch in the workspace) in a batch with small ones
+    matches the single-transform entry point; an in-place two-pass job is refused."""
+    from halo2_amd import _lib, device as dv, fields as F, ntt, msm
+
+    ks = [25, 4, 14]
+    print("fft batch with 2^25", flush=True)
+    d_in = [msm.random_fr(1 << k, 1500 + k) for k in ks]
+    outs = ntt.fft_batch_to_device(1, d_in, ks, ks, [F.omega(k) for k in ks], [1] * len(ks))
+    for k, d, o in zip(ks, d_in, outs):
+        ref = dv.empty_u64(4 << k)
+        ntt.fft_normal_device(1, k, d, ref, F.omega(k), 1)
+        assert bool((ref == o).all()), f"k={k}"
+    small = rand_fr(co, 1 << 4, 3)
+    assert (dv.to_host(outs[1]) == _oracle_kind(co, po, 1, dv.to_host(d_in[1]), 4)).all()
+    d = dv.to_device(rand_fr(co, 1 << 14, 4))
+    with pytest.raises(_lib.Halo2GpuError):
+        ntt.fft_batch_to_device(1, [d], [14], [14], [F.omega(14)], [1], outputs=[d])
+    # a one-pass transform may run in place
+    d2 = dv.to_device(small)
+    ntt.fft_batch_to_device(1, [d2], [4], [4], [F.omega(4)], [1], outputs=[d2])
+    assert (dv.to_host(d2) == _oracle_kind(co, po, 1, small, 4)).all()
+
+
+def test_coeff_to_extended_batch(lib, co, po):
+    """Domains of k = 4 .. 16 (extended k + 2): the batched coeff_to_extended equals each
+    domain's own."""
+    from halo2_amd import device as dv
+    from halo2_amd.domain import EvaluationDomain, coeff_to_extended_batch_device
+
+    ks = [16, 4, 9, 12, 13, 10, 16]
+    doms = [EvaluationDomain(5, k) for k in ks]
+    polys = [dv.to_device(rand_fr(co, 1 << k, 1600 + i)) for i, k in enumerate(ks)]
+    outs = coeff_to_extended_batch_device(doms, polys)
+    for dom, p, o in zip(doms, polys, outs):
+        assert bool((dom.coeff_to_extended_many_device([p])[0] == o).all()), f"k={dom.k}"
