@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--no-msm", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rows", action="store_true", help="skip the per-row measurements (products, quotient, ...)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the uneven batch (32 commit MSMs + coeff_to_extended, per poly vs batched)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one plain device copy of this size first (FETCH_SIZE/WRITE_SIZE calibration)")
     return ap.parse_args()
@@ -154,6 +156,55 @@ def row_measurements(lib, dv, F, msm, torch, stream) -> dict:
     pts = msm.g1_mul_generator(msm.random_fr(m, seed=350), m)
     scal = msm.random_fr(m, seed=351)
     rec("msm_2^16", m, _timed(lambda: msm.multiexp_device_scalars_device_bases(scal, pts, m), 5, torch), 96)
+    return out
+
+
+def batch_measurements(msm, torch, seed: int = 2024, count: int = 32) -> dict:
+    """Uneven batch: `count` polynomials with k drawn from {14..22}, each with its KZG
+    commit MSM (against one SRS) and its coeff_to_extended (quotient degree 4), run one
+    call per polynomial and as one batched pipeline each; the results are compared."""
+    import numpy as np
+
+    from halo2_amd.domain import EvaluationDomain, coeff_to_extended_batch_device
+
+    rng = np.random.default_rng(seed)
+    ks = [int(x) for x in rng.integers(14, 23, count)]
+    nmax = 1 << max(ks)
+    srs = msm.g1_mul_generator(msm.random_fr(nmax, seed=seed + 1), nmax)
+    polys = [msm.random_fr(1 << k, seed=seed + 10 + i) for i, k in enumerate(ks)]
+    doms = [EvaluationDomain(5, k) for k in ks]
+
+    def msm_one():
+        return [msm.multiexp_device_scalars_device_bases(p, srs, p.numel() // 4) for p in polys]
+
+    def msm_batch():
+        return msm.multiexp_batch_device(polys, srs)
+
+    def c2e_one():
+        return [d.coeff_to_extended_many_device([p])[0] for d, p in zip(doms, polys)]
+
+    def c2e_batch():
+        return coeff_to_extended_batch_device(doms, polys)
+
+    a, b = msm_one(), msm_batch()
+    same_msm = all((a[i] == b[i]).all() for i in range(count))
+    a, b = c2e_one(), c2e_batch()
+    same_c2e = all(bool((x == y).all()) for x, y in zip(a, b))
+    del a, b
+    out = {"ks": ks, "elements": int(sum(1 << k for k in ks)), "same_results": bool(same_msm and same_c2e)}
+    for name, f in (("msm_per_poly_ms", msm_one), ("msm_batched_ms", msm_batch), ("c2e_per_poly_ms", c2e_one),
+                    ("c2e_batched_ms", c2e_batch)):
+        ts = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            ta = time.perf_counter()
+            r = f()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - ta)
+            del r
+        out[name] = round(min(ts) * 1e3, 2)
+    out["batched_speedup"] = round((out["msm_per_poly_ms"] + out["c2e_per_poly_ms"])
+                                   / (out["msm_batched_ms"] + out["c2e_batched_ms"]), 3)
     return out
 
 
@@ -305,6 +356,10 @@ def main() -> int:
     # ----------------------------------------- the other hot-path rows
     if not args.no_rows:
         extra["rows"] = row_measurements(lib, dv, F, msm, torch, stream)
+
+    # ------------------------------- uneven batch (32 polys, k in 14..22)
+    if not args.no_batch:
+        extra["batch_uneven_32"] = batch_measurements(msm, torch)
 
     # ------------------------------------------------------ CPU baseline
     cpu = None

@@ -172,7 +172,9 @@ class ParamsKZG:
     Lagrange basis g_lagrange[i] = L_i(tau) G = G * ((tau^n - 1)/n) * w^i / (tau - w^i),
     both generated on the device. The lazy device mirrors of the reference
     (`g_device`, `g_lagrange_device`, :246-304) are the resident buffers themselves.
-    G2 elements belong to the verifier and are not part of this path."""
+    g2 and s_g2 = tau G2 (affine over Fq2, Python integers) are computed on the host
+    when first asked for; `from_parts` builds the object from stored points (an SRS
+    file, halo2_amd.srs)."""
 
     def __init__(self, k: int, tau: int, omega: int | None = None, fixed_base: bool = True):
         from .fields import omega as root
@@ -186,6 +188,43 @@ class ParamsKZG:
         # fixed-base tables (rows 2^(c w) of each basis), built on first commit
         self.fixed_base = fixed_base
         self._tables: dict[str, torch.Tensor] = {}
+        self._g2 = self._s_g2 = None
+
+    @classmethod
+    def from_parts(cls, k: int, g_device: torch.Tensor, g_lagrange_device: torch.Tensor, g2, s_g2,
+                   fixed_base: bool = True) -> "ParamsKZG":
+        """commitment.rs:205-228 with both bases given (device affine points, n each)."""
+        from .fields import omega as root
+
+        assert g_device.numel() == 8 << k and g_lagrange_device.numel() == 8 << k
+        self = cls.__new__(cls)
+        self.k, self.n = k, 1 << k
+        self.tau = None
+        self.omega = root(k)
+        self.g_device = g_device
+        self._g_lagrange_device = g_lagrange_device
+        self.fixed_base = fixed_base
+        self._tables = {}
+        self._g2, self._s_g2 = g2, s_g2
+        return self
+
+    @property
+    def g2(self):
+        """commitment.rs:306-309: the G2 generator."""
+        if self._g2 is None and self.tau is not None:
+            from .srs import G2_GENERATOR
+
+            self._g2 = G2_GENERATOR
+        return self._g2
+
+    @property
+    def s_g2(self):
+        """commitment.rs:311-314: tau G2 (setup, :191-192)."""
+        if self._s_g2 is None and self.tau is not None:
+            from .srs import g2_mul
+
+            self._s_g2 = g2_mul(self.g2, self.tau)
+        return self._s_g2
 
     def _commit_resident(self, d_scalars: torch.Tensor, which: str) -> np.ndarray:
         n = d_scalars.numel() // 4

@@ -422,6 +422,22 @@ uint64_t h2amd_multiexp_batch_workspace_size(const uint64_t* lengths, uint32_t c
 RustError h2amd_multiexp_batch(const void* const* d_scalars, const uint64_t* lengths, uint32_t count,
     const void* d_bases, void* h_out, void* scratch, uint64_t scratch_bytes, void* stream);
 
+/* ======================= SRS point formats (G1 halves of ParamsKZG::write_custom /
+ * read_custom, src/poly/kzg/commitment.rs:317-414; no FFI counterpart: the reference
+ * decodes on the host). Points are device affine 64-byte (x, y) Montgomery words.
+ * Validation (SerdeFormat::RawBytes): a coordinate >= p or a point off y^2 = x^3 + 3 is
+ * invalid; the identity (0, 0) is valid. Compressed (SerdeFormat::Processed): 32 bytes,
+ * canonical x little-endian with the parity of canonical y in bit 7 of byte 31, all-zero
+ * = identity. Validation and decompression report the invalid count and the first
+ * invalid index (~0 when none) in host words and synchronise; they need
+ * h2amd_g1_flags_workspace_size() bytes of device workspace. */
+uint64_t h2amd_g1_flags_workspace_size(void);
+RustError h2amd_g1_validate(const void* d_points, uint64_t n, uint64_t* h_invalid, uint64_t* h_first_invalid,
+    void* scratch, uint64_t scratch_bytes, void* stream);
+RustError h2amd_g1_compress(void* d_out, const void* d_points, uint64_t n, void* stream);
+RustError h2amd_g1_decompress(void* d_points, const void* d_in, uint64_t n, uint64_t* h_invalid,
+    uint64_t* h_first_invalid, void* scratch, uint64_t scratch_bytes, void* stream);
+
 /* ======================= kernel timing (no reference counterpart)
  * When enabled, launchers bracket their hot kernels ("ntt_pass", "msm_bucket_level0",
  * ...) with HIP events on the launch stream; reading syncs the recorded events. */
