@@ -69,14 +69,14 @@ struct G1X29 {
     __device__ __forceinline__ static G1X29 dbl_affine(const G1A29& a)
     {
         const Fq29 U = Fq29::add(a.y, a.y); // < 4, limbs < 2^30
-        const Fq29 V = Fq29::mul(U, U); // < 1.1
+        const Fq29 V = Fq29::sqr(U); // < 1.1
         const Fq29 W = Fq29::mul(U, V);
         const Fq29 S = Fq29::mul(a.x, V);
-        const Fq29 x2 = Fq29::mul(a.x, a.x);
+        const Fq29 x2 = Fq29::sqr(a.x);
         const Fq29 M = Fq29::normalize(Fq29::add(Fq29::add(x2, x2), x2)); // < 3.3
         const Fq29 S2 = Fq29::normalize(Fq29::add(S, S)); // < 2.2
         G1X29 r;
-        r.X = Fq29::reduce_2m(Fq29::normalize(Fq29::sub4(Fq29::mul(M, M), S2)));
+        r.X = Fq29::reduce_2m(Fq29::normalize(Fq29::sub4(Fq29::sqr(M), S2)));
         const Fq29 t = Fq29::mul(M, nsub2(S, r.X));
         r.Y = Fq29::reduce_2m(nsub2(t, Fq29::mul(W, a.y)));
         r.ZZ = V;
@@ -92,14 +92,14 @@ struct G1X29 {
         if (p.is_inf())
             return p;
         const Fq29 U = Fq29::add(p.Y, p.Y); // < 4, limbs < 2^30
-        const Fq29 V = Fq29::mul(U, U);
+        const Fq29 V = Fq29::sqr(U);
         const Fq29 W = Fq29::mul(U, V);
         const Fq29 S = Fq29::mul(p.X, V);
-        const Fq29 x2 = Fq29::mul(p.X, p.X);
+        const Fq29 x2 = Fq29::sqr(p.X);
         const Fq29 M = Fq29::normalize(Fq29::add(Fq29::add(x2, x2), x2));
         const Fq29 S2 = Fq29::normalize(Fq29::add(S, S));
         G1X29 r;
-        r.X = Fq29::reduce_2m(Fq29::normalize(Fq29::sub4(Fq29::mul(M, M), S2)));
+        r.X = Fq29::reduce_2m(Fq29::normalize(Fq29::sub4(Fq29::sqr(M), S2)));
         const Fq29 t = Fq29::mul(M, nsub2(S, r.X));
         r.Y = Fq29::reduce_2m(nsub2(t, Fq29::mul(W, p.Y)));
         r.ZZ = Fq29::mul(V, p.ZZ);
@@ -123,12 +123,12 @@ struct G1X29 {
                 return dbl_affine(q);
             return inf();
         }
-        const Fq29 PP = Fq29::mul(P, P);
+        const Fq29 PP = Fq29::sqr(P);
         const Fq29 PPP = Fq29::mul(P, PP);
         const Fq29 Q = Fq29::mul(p.X, PP);
         const Fq29 T = Fq29::normalize(Fq29::add(PPP, Fq29::add(Q, Q))); // < 3.2
         G1X29 r;
-        r.X = Fq29::reduce_2m(Fq29::normalize(Fq29::sub4(Fq29::mul(R, R), T)));
+        r.X = Fq29::reduce_2m(Fq29::normalize(Fq29::sub4(Fq29::sqr(R), T)));
         const Fq29 t = Fq29::mul(R, nsub2(Q, r.X));
         r.Y = Fq29::reduce_2m(nsub2(t, Fq29::mul(p.Y, PPP)));
         r.ZZ = Fq29::mul(p.ZZ, PP);
@@ -154,17 +154,31 @@ struct G1X29 {
                 return dbl(p);
             return inf();
         }
-        const Fq29 PP = Fq29::mul(P, P);
+        const Fq29 PP = Fq29::sqr(P);
         const Fq29 PPP = Fq29::mul(P, PP);
         const Fq29 Q = Fq29::mul(U1, PP);
         const Fq29 T = Fq29::normalize(Fq29::add(PPP, Fq29::add(Q, Q)));
         G1X29 r;
-        r.X = Fq29::reduce_2m(Fq29::normalize(Fq29::sub4(Fq29::mul(R, R), T)));
+        r.X = Fq29::reduce_2m(Fq29::normalize(Fq29::sub4(Fq29::sqr(R), T)));
         const Fq29 t = Fq29::mul(R, nsub2(Q, r.X));
         r.Y = Fq29::reduce_2m(nsub2(t, Fq29::mul(S1, PPP)));
         r.ZZ = Fq29::mul(Fq29::mul(p.ZZ, q.ZZ), PP);
         r.ZZZ = Fq29::mul(Fq29::mul(p.ZZZ, q.ZZZ), PPP);
         return r;
+    }
+
+    // XYZZ -> Jacobian (X ZZ^2, Y ZZZ^2, ZZZ): x = X ZZ^2 / ZZZ^2 = X / ZZ and
+    // y = Y ZZZ^2 / ZZZ^3 because ZZ^3 = ZZZ^2; the identity keeps Z = 0
+    __device__ __forceinline__ void to_jacobian(Fq29& jx, Fq29& jy, Fq29& jz) const
+    {
+        jx = Fq29::mul(X, Fq29::sqr(ZZ));
+        jy = Fq29::mul(Y, Fq29::sqr(ZZZ));
+        jz = ZZZ;
+    }
+    __device__ __forceinline__ static G1X29 from_jacobian(const Fq29& jx, const Fq29& jy, const Fq29& jz)
+    {
+        const Fq29 zz = Fq29::sqr(jz);
+        return G1X29 { jx, jy, zz, Fq29::mul(zz, jz) };
     }
 
     // back to the 2^256 scale (canonical) for the final affine conversion
@@ -179,5 +193,29 @@ struct G1X29 {
         return r;
     }
 };
+
+// Jacobian doubling for long doubling chains (the Horner over MSM windows):
+// dbl-2009-l for a = 0, 2M + 5S against XYZZ's 6M + 3S. Coordinates < 2m,
+// normalised, in and out; Z = 0 is the identity and stays it.
+__device__ __forceinline__ void jacobian_dbl(Fq29& X, Fq29& Y, Fq29& Z)
+{
+    const Fq29 A = Fq29::sqr(X); // < 1.01
+    const Fq29 B = Fq29::sqr(Y);
+    const Fq29 C = Fq29::sqr(B);
+    const Fq29 T = Fq29::sqr(Fq29::add(X, B)); // (X + B)^2, operand < 3.01, limbs < 2^30
+    const Fq29 AC = Fq29::normalize(Fq29::add(A, C)); // < 2.02
+    const Fq29 Dh = Fq29::reduce_2m(Fq29::normalize(Fq29::sub4(T, AC))); // (X + B)^2 - A - C, < 2
+    const Fq29 D = Fq29::reduce_2m(Fq29::normalize(Fq29::add(Dh, Dh))); // < 2
+    const Fq29 E = Fq29::normalize(Fq29::add(Fq29::add(A, A), A)); // 3A < 3.03
+    const Fq29 F = Fq29::sqr(E); // < 1.01
+    const Fq29 D2 = Fq29::normalize(Fq29::add(D, D)); // < 4
+    const Fq29 X3 = Fq29::reduce_2m(Fq29::normalize(Fq29::sub4(F, D2))); // F - 2D, < 2
+    const Fq29 t = Fq29::mul(E, nsub2(D, X3)); // E (D - X3), < 1.01
+    const Fq29 C2 = Fq29::add(C, C);
+    const Fq29 C8 = Fq29::reduce_2m(Fq29::normalize(Fq29::add(Fq29::add(C2, C2), Fq29::add(C2, C2)))); // 8C, < 2
+    Z = Fq29::mul(Fq29::add(Y, Y), Z); // 2 Y Z: operand limbs < 2^30, < 1.01
+    Y = Fq29::reduce_2m(nsub2(t, C8)); // E (D - X3) - 8C: < 3.01 -> < 2
+    X = X3;
+}
 
 } // namespace h2

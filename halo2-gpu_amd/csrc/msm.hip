@@ -909,12 +909,16 @@ __global__ void __launch_bounds__(64) finalize_kernel(const PointSlot* __restric
         return;
     win += (uint64_t)j * W;
     out += 12ull * j;
-    G1X29 acc = G1X29::load(win + (W - 1));
+    // the c doublings per window run in Jacobian coordinates (7 products per
+    // doubling instead of XYZZ's 9), the window additions in XYZZ
+    Fq29 X, Y, Z;
+    G1X29::load(win + (W - 1)).to_jacobian(X, Y, Z);
     for (int w = (int)W - 2; w >= 0; --w) {
         for (uint32_t k = 0; k < c; ++k)
-            acc = G1X29::dbl_inline(acc);
-        acc = G1X29::add(acc, G1X29::load(win + w));
+            jacobian_dbl(X, Y, Z);
+        G1X29::add(G1X29::from_jacobian(X, Y, Z), G1X29::load(win + w)).to_jacobian(X, Y, Z);
     }
+    const G1X29 acc = G1X29::from_jacobian(X, Y, Z);
     Fq one = Fq::one();
     if (acc.is_inf()) {
         Fq::zero().store(out);
@@ -967,7 +971,7 @@ __global__ void window_affine_kernel(const PointSlot* __restrict__ xyzz, const u
     const Fq29 iz3 = Fq29::from_fp(Fq::load(inv + 8 * i) * k); // 1/ZZZ in the 2^261 scale
     const G1X29 p = G1X29::load(xyzz + i);
     const Fq29 iz = Fq29::mul(p.ZZ, iz3); // 1/Z
-    const Fq29 iz2 = Fq29::mul(iz, iz);
+    const Fq29 iz2 = Fq29::sqr(iz);
     Fq29::reduce_full(Fq29::mul(p.X, iz2)).store32(row + 64 * i);
     Fq29::reduce_full(Fq29::mul(p.Y, iz3)).store32(row + 64 * i + 32);
 }
