@@ -976,6 +976,116 @@ __global__ void window_affine_kernel(const PointSlot* __restrict__ xyzz, const u
     Fq29::reduce_full(Fq29::mul(p.Y, iz3)).store32(row + 64 * i + 32);
 }
 
+// ------------------------------------------------- Lagrange bases from g
+//
+// g_lagrange[i] = n^-1 sum_j omega^-ij g_j (an inverse NTT over G1 points, as
+// g_to_lagrange does with best_fft): radix-2 decimation in time after a
+// bit-reversed load, every twiddle applied as a scalar multiplication of the
+// odd input (2-bit windows, most significant first), then the n^-1 scaling and
+// one batched inversion for the affine form.
+
+// twiddles: tw[i] = canonical(omega^i), i < count (omega in Montgomery form)
+__global__ void ec_twiddle_kernel(Fr omega, uint32_t* __restrict__ tw, uint64_t count)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count)
+        return;
+    Fr::from_mont(Fr::pow_u64(omega, i)).store(tw + 8 * i);
+}
+
+// P[bitrev(i)] = g_i as XYZZ in the 2^261 scale
+__global__ void ec_load_kernel(const uint8_t* __restrict__ g, PointSlot* __restrict__ P, uint64_t n, uint32_t k)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    Fq t;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        t.v[q] = Fq29Params::TO261[q];
+    const Fq x = Fq::load(g + 64 * i) * t, y = Fq::load(g + 64 * i + 32) * t; // (0, 0) stays (0, 0)
+    const G1A29 a { Fq29::from_fp(x), Fq29::from_fp(y) };
+    const uint64_t r = k ? (uint64_t)(__brevll(i) >> (64 - k)) : 0;
+    G1X29::from_affine(a).store(P + r);
+}
+
+// s * p for a canonical scalar s (< 2^254), 2-bit windows from the top
+__device__ G1X29 scalar_mul(const G1X29& p, const uint32_t (&s)[8])
+{
+    const G1X29 p2 = G1X29::dbl(p);
+    const G1X29 p3 = G1X29::add(p2, p);
+    G1X29 acc = G1X29::inf();
+    for (int w = 126; w >= 0; --w) {
+        acc = G1X29::dbl_inline(G1X29::dbl_inline(acc));
+        const uint32_t d = (s[w >> 4] >> ((w & 15) * 2)) & 3u;
+        if (d)
+            acc = G1X29::add(acc, d == 1 ? p : (d == 2 ? p2 : p3));
+    }
+    return acc;
+}
+
+__device__ __forceinline__ G1X29 g1x_neg(const G1X29& p)
+{
+    return G1X29 { p.X, nsub2(Fq29::zero(), p.Y), p.ZZ, p.ZZZ };
+}
+
+// stage s of the transform: butterflies (a + w b, a - w b) at distance 2^s,
+// w = omega^(j 2^(k-1-s)) for position j inside the half block
+__global__ void ec_stage_kernel(PointSlot* __restrict__ P, const uint32_t* __restrict__ tw, uint64_t n, uint32_t k,
+    uint32_t s)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n / 2)
+        return;
+    const uint64_t half = 1ull << s;
+    const uint64_t j = t & (half - 1);
+    const uint64_t i0 = ((t >> s) << (s + 1)) | j;
+    const uint64_t i1 = i0 + half;
+    const G1X29 a = G1X29::load(P + i0);
+    G1X29 b = G1X29::load(P + i1);
+    if (j) {
+        uint32_t w[8];
+        const uint4* q = (const uint4*)(tw + 8 * (j << (k - 1 - s)));
+        const uint4 lo = q[0], hi = q[1];
+        w[0] = lo.x; w[1] = lo.y; w[2] = lo.z; w[3] = lo.w;
+        w[4] = hi.x; w[5] = hi.y; w[6] = hi.z; w[7] = hi.w;
+        b = scalar_mul(b, w);
+    }
+    G1X29::add(a, b).store(P + i0);
+    G1X29::add(a, g1x_neg(b)).store(P + i1);
+}
+
+// P[i] *= s (canonical scalar) and zzz[i] = ZZZ in the 2^256 scale for the inversion
+__global__ void ec_scale_kernel(PointSlot* __restrict__ P, uint32_t* __restrict__ zzz, uint64_t n, Fr s)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const G1X29 r = scalar_mul(G1X29::load(P + i), s.v);
+    r.store(P + i);
+    Fq29::reduce_full(Fq29::mul(r.ZZZ, Fq29::from_limbs(Fq29P::FROM261))).store32(zzz + 8 * i);
+}
+
+// out[i] = affine(P[i]) in the boundary's 2^256 scale, with inv[i] = 1/ZZZ
+// (2^256-scale words; 0 for the identity, which gives (0, 0))
+__global__ void ec_affine_kernel(const PointSlot* __restrict__ P, const uint32_t* __restrict__ inv,
+    uint8_t* __restrict__ out, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    Fq k;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        k.v[q] = Fq29Params::TO261[q];
+    const Fq29 iz3 = Fq29::from_fp(Fq::load(inv + 8 * i) * k); // 1/ZZZ in the 2^261 scale
+    const G1X29 p = G1X29::load(P + i);
+    const Fq29 iz = Fq29::mul(p.ZZ, iz3); // 1/Z
+    const Fq29 from = Fq29::from_limbs(Fq29P::FROM261);
+    Fq29::reduce_full(Fq29::mul(Fq29::mul(p.X, Fq29::sqr(iz)), from)).store32(out + 64 * i);
+    Fq29::reduce_full(Fq29::mul(Fq29::mul(p.Y, iz3), from)).store32(out + 64 * i + 32);
+}
+
 unsigned blocks_for(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
 uint32_t ilog2(uint32_t x) { return 31 - __builtin_clz(x); }
@@ -1282,6 +1392,74 @@ RustError h2amd_multiexp_batch(const void* const* d_scalars, const uint64_t* len
     H2_TRY(hipMemcpyAsync(o, b.out, 96ull * count, hipMemcpyDeviceToHost, stream));
     // also keeps the host job tables alive until their copies are done
     H2_TRY(hipStreamSynchronize(stream));
+    return ok();
+}
+
+// ----------------------------------- Lagrange bases (no reference counterpart)
+
+uint64_t h2amd_g1_to_lagrange_workspace_size(uint32_t k)
+{
+    if (k > 28)
+        return 0;
+    const uint64_t n = 1ull << k;
+    WorkspaceSizer w;
+    w.take(128 * n);
+    w.take(32 * n);
+    w.take(32 * (n / 2 ? n / 2 : 1));
+    return w.bytes + 1024;
+}
+
+RustError h2amd_g1_to_lagrange(void* d_out, const void* d_g, uint32_t k, const void* h_omega_inv, void* scratch,
+    uint64_t scratch_bytes, void* stream_)
+{
+    hipStream_t stream = (hipStream_t)stream_;
+    H2_CHECK_ARG(k <= 28, "h2amd_g1_to_lagrange: k=%u exceeds 28", k);
+    const uint64_t n = 1ull << k;
+    Workspace ws(scratch, scratch_bytes);
+    PointSlot* P = (PointSlot*)ws.take(128 * n);
+    uint32_t* zzz = (uint32_t*)ws.take(32 * n);
+    uint32_t* tw = (uint32_t*)ws.take(32 * (n / 2 ? n / 2 : 1));
+    if (ws.overflow)
+        return fail((int)hipErrorInvalidValue, "h2amd_g1_to_lagrange: workspace of %lu bytes is too small",
+            (unsigned long)scratch_bytes);
+    Fr omega;
+    memcpy(omega.v, h_omega_inv, 32);
+    // n^-1 = (2^-1)^k mod r, canonical, on the host
+    const uint32_t* M = FrParams::M;
+    uint32_t inv[8] = { 1, 0, 0, 0, 0, 0, 0, 0 };
+    for (uint32_t i = 0; i < k; ++i) { // inv = inv / 2 mod r
+        const bool odd = inv[0] & 1u;
+        uint64_t c = 0;
+        uint32_t t[9];
+        for (int q = 0; q < 8; ++q) {
+            c += (uint64_t)inv[q] + (odd ? M[q] : 0u);
+            t[q] = (uint32_t)c;
+            c >>= 32;
+        }
+        t[8] = (uint32_t)c;
+        for (int q = 0; q < 8; ++q)
+            inv[q] = (t[q] >> 1) | (t[q + 1] << 31);
+    }
+    Fr ninv;
+    memcpy(ninv.v, inv, 32);
+    const unsigned T = 256;
+    if (n > 1) {
+        ec_twiddle_kernel<<<blocks_for(n / 2, T), T, 0, stream>>>(omega, tw, n / 2);
+        H2_LAUNCH_CHECK();
+    }
+    ec_load_kernel<<<blocks_for(n, T), T, 0, stream>>>((const uint8_t*)d_g, P, n, k);
+    H2_LAUNCH_CHECK();
+    for (uint32_t s = 0; s < k; ++s) {
+        ec_stage_kernel<<<blocks_for(n / 2, T), T, 0, stream>>>(P, tw, n, k, s);
+        H2_LAUNCH_CHECK();
+    }
+    ec_scale_kernel<<<blocks_for(n, T), T, 0, stream>>>(P, zzz, n, ninv);
+    H2_LAUNCH_CHECK();
+    RustError e = _halo2_batch_invert(zzz, 1, n, stream);
+    if (e.code)
+        return e;
+    ec_affine_kernel<<<blocks_for(n, T), T, 0, stream>>>(P, zzz, (uint8_t*)d_out, n);
+    H2_LAUNCH_CHECK();
     return ok();
 }
 

@@ -144,6 +144,8 @@ SIGNATURES = {
     "h2amd_multiexp_precomputed": (RustError, [P, P, U64, OBJP, U64, P, U64, P]),
     "h2amd_multiexp_batch_workspace_size": (U64, [P, U32]),
     "h2amd_g1_flags_workspace_size": (U64, []),
+    "h2amd_g1_to_lagrange_workspace_size": (U64, [U32]),
+    "h2amd_g1_to_lagrange": (RustError, [P, P, U32, P, P, U64, P]),
     "h2amd_g1_validate": (RustError, [P, U64, P, P, P, U64, P]),
     "h2amd_g1_compress": (RustError, [P, P, U64, P]),
     "h2amd_g1_decompress": (RustError, [P, P, U64, P, P, P, U64, P]),

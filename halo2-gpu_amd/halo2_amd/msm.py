@@ -138,6 +138,25 @@ def multiexp_batch_device(d_scalars: list[torch.Tensor], d_bases: torch.Tensor,
     return out[:count]
 
 
+def g_to_lagrange_device(d_g: torch.Tensor, k: int, omega: int | None = None) -> torch.Tensor:
+    """`h2amd_g1_to_lagrange` (halo2-axiom g_to_lagrange, used by ParamsKZG::from_parts and
+    downsize, commitment.rs:205-228, :436-450): the Lagrange bases n^-1 sum_j w^-ij g_j of
+    2^k device affine points, w the 2^k-th root of unity (default: the domain's)."""
+    from .fields import inv, omega as root
+
+    lib = _lib.load()
+    n = 1 << k
+    assert d_g.numel() >= 8 * n
+    w_inv = inv(root(k) if omega is None else omega)
+    wb = np.ascontiguousarray(mont_limbs(w_inv))
+    out = dv.empty_u64(8 * n)
+    ws = int(lib.h2amd_g1_to_lagrange_workspace_size(k))
+    scratch = dv.empty_bytes(ws)
+    _lib.check(lib.h2amd_g1_to_lagrange(dv.ptr(out), dv.ptr(d_g), k, wb.ctypes.data, dv.ptr(scratch), ws,
+                                        dv.stream()), "h2amd_g1_to_lagrange")
+    return out
+
+
 def g1_mul_generator(d_scalars: torch.Tensor, n: int) -> torch.Tensor:
     """d_out[i] = s_i * G (affine, 8 words per point) on the device."""
     lib = _lib.load()
@@ -191,12 +210,16 @@ class ParamsKZG:
         self._g2 = self._s_g2 = None
 
     @classmethod
-    def from_parts(cls, k: int, g_device: torch.Tensor, g_lagrange_device: torch.Tensor, g2, s_g2,
+    def from_parts(cls, k: int, g_device: torch.Tensor, g_lagrange_device: torch.Tensor | None, g2, s_g2,
                    fixed_base: bool = True) -> "ParamsKZG":
-        """commitment.rs:205-228 with both bases given (device affine points, n each)."""
+        """commitment.rs:205-228: from device affine bases (n each); without g_lagrange it is
+        computed from g (g_to_lagrange), as the reference does."""
         from .fields import omega as root
 
-        assert g_device.numel() == 8 << k and g_lagrange_device.numel() == 8 << k
+        assert g_device.numel() == 8 << k
+        if g_lagrange_device is None:  # from_parts without g_lagrange: g_to_lagrange
+            g_lagrange_device = g_to_lagrange_device(g_device, k)
+        assert g_lagrange_device.numel() == 8 << k
         self = cls.__new__(cls)
         self.k, self.n = k, 1 << k
         self.tau = None
@@ -207,6 +230,18 @@ class ParamsKZG:
         self._tables = {}
         self._g2, self._s_g2 = g2, s_g2
         return self
+
+    def downsize(self, k: int) -> None:
+        """commitment.rs:436-450: keep the first 2^k bases of g and rebuild g_lagrange for
+        the smaller domain (g_to_lagrange of the truncated g); cached tables are dropped."""
+        from .fields import omega as root
+
+        assert k <= self.k
+        self.k, self.n = k, 1 << k
+        self.omega = root(k)
+        self.g_device = self.g_device[: 8 * self.n].clone()
+        self._g_lagrange_device = g_to_lagrange_device(self.g_device, k)
+        self._tables = {}
 
     @property
     def g2(self):

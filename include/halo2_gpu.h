@@ -438,6 +438,15 @@ RustError h2amd_g1_compress(void* d_out, const void* d_points, uint64_t n, void*
 RustError h2amd_g1_decompress(void* d_points, const void* d_in, uint64_t n, uint64_t* h_invalid,
     uint64_t* h_first_invalid, void* scratch, uint64_t scratch_bytes, void* stream);
 
+/* ======================= Lagrange bases from coefficient bases (no FFI counterpart:
+ * the reference runs halo2-axiom's g_to_lagrange on the host, from ParamsKZG::from_parts and
+ * downsize, src/poly/kzg/commitment.rs:205-228 and :436-450). d_out[i] = n^-1 sum_j
+ * omega_inv^(i j) d_g[j] for n = 2^k device affine points, omega_inv (host Montgomery word)
+ * the inverse of the 2^k-th root of unity; affine output, identity (0, 0). Enqueues only. */
+uint64_t h2amd_g1_to_lagrange_workspace_size(uint32_t k);
+RustError h2amd_g1_to_lagrange(void* d_out, const void* d_g, uint32_t k, const void* h_omega_inv, void* scratch,
+    uint64_t scratch_bytes, void* stream);
+
 /* ======================= kernel timing (no reference counterpart)
  * When enabled, launchers bracket their hot kernels ("ntt_pass", "msm_bucket_level0",
  * ...) with HIP events on the launch stream; reading syncs the recorded events. */

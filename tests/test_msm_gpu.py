@@ -369,3 +369,40 @@ def test_kzg_commit_many(lib, co):
         exp = co.g1_mul(co.g1_generator(), co.fr_inner_product(p, pows[: len(p)]))
         assert (got[j] == exp).all(), f"poly {j}"
         assert (params.commit_device(dv.to_device(p)) == exp).all()
+
+
+@pytest.mark.parametrize("k", [0, 1, 2, 5, 8, 11])
+def test_g_to_lagrange_matches_setup(lib, co, k):
+    """g_to_lagrange(g) on the device equals the Lagrange bases the setup builds
+    directly from tau (L_i(tau) G, commitment.rs:157-183): two independent routes."""
+    from halo2_amd import device as dv, fields as F, msm
+
+    print(f"g_to_lagrange k={k}", flush=True)
+    params = msm.ParamsKZG(k, 0x7A0_5EED % F.R_MOD, fixed_base=False)
+    got = msm.g_to_lagrange_device(params.g_device, k)
+    assert (dv.to_host(got, cols=8) == dv.to_host(params.g_lagrange_device, cols=8)).all()
+
+
+def test_g_to_lagrange_identities_and_downsize(lib, co):
+    """Bases (G, O, O, ...): every output is n^-1 G (identity inputs through every stage);
+    downsize(k) equals a fresh setup at k; from_parts without g_lagrange computes it."""
+    from halo2_amd import device as dv, fields as F, msm
+
+    k = 6
+    n = 1 << k
+    g = np.zeros((n, 8), dtype=np.uint64)
+    g[0] = co.g1_generator()
+    out = dv.to_host(msm.g_to_lagrange_device(dv.to_device(g), k), cols=8)
+    exp = co.g1_mul(co.g1_generator(), F.mont_limbs(F.inv(n)))
+    for i in range(n):
+        assert (out[i] == exp[:8]).all(), i
+    tau = 0x1234_5678_9ABC % F.R_MOD
+    big = msm.ParamsKZG(12, tau, fixed_base=False)
+    big.downsize(9)
+    small = msm.ParamsKZG(9, tau, fixed_base=False)
+    assert big.n == 512 and (dv.to_host(big.g_device, cols=8) == dv.to_host(small.g_device, cols=8)).all()
+    assert (dv.to_host(big.g_lagrange_device, cols=8) == dv.to_host(small.g_lagrange_device, cols=8)).all()
+    parts = msm.ParamsKZG.from_parts(9, small.g_device, None, small.g2, small.s_g2, fixed_base=False)
+    assert (dv.to_host(parts.g_lagrange_device, cols=8) == dv.to_host(small.g_lagrange_device, cols=8)).all()
+    poly = rand_fr(co, 512, 3)
+    assert (parts.commit_lagrange(poly) == small.commit_lagrange(poly)).all()
