@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--no-msm", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rows", action="store_true", help="skip the per-row measurements (products, quotient, ...)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the per-size MSM width / NTT tile measurements against the default rules")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the uneven batch (32 commit MSMs + coeff_to_extended, per poly vs batched)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -360,6 +362,19 @@ def main() -> int:
     # ------------------------------- uneven batch (32 polys, k in 14..22)
     if not args.no_batch:
         extra["batch_uneven_32"] = batch_measurements(msm, torch)
+
+    # ------------------ tuner: MSM width / NTT tile per size vs the default rules
+    if not args.no_autotune:
+        from halo2_amd import autotune
+
+        tuned = autotune.tune(msm_logs=(16, 20, 22, 24), ntt_logs=(16, 18, 20, 22), apply=False)
+        extra["autotune"] = {
+            "msm": {f"2^{lg}": {"default_c": v["default"], "best_c": v["best"],
+                                "default_ms": v["ms"][v["default"]], "best_ms": v["ms"][v["best"]]}
+                    for lg, v in tuned["msm"].items()},
+            "ntt": {f"2^{lg}": {"best_tile_log2": v["best"], "default_ms": v["ms"][0],
+                                "best_ms": v["ms"][v["best"]]} for lg, v in tuned["ntt"].items()},
+        }
 
     # ------------------------------------------------------ CPU baseline
     cpu = None

@@ -103,8 +103,17 @@ uint32_t log2_floor(uint64_t n)
 // Window width: plain bases need W bucket sets (one per window) and a Horner
 // over windows, so c grows more slowly than log2 n; precomputed bases
 // (rows 2^(c w) P) share one bucket set, so the width can follow log2 n.
+// Window widths chosen by a tuner, per floor(log2 n) of a plain MSM (0 = use
+// the default rule). Set with h2amd_msm_set_widths before sizing and launching.
+uint8_t g_width_table[32] = { 0 };
+
 uint32_t plan_width(uint64_t n, bool shared)
 {
+    if (!shared) {
+        const uint32_t tuned = g_width_table[log2_floor(n < 1 ? 1 : n) & 31];
+        if (tuned)
+            return tuned;
+    }
     int c = (int)log2_floor(n < 1 ? 1 : n) - (shared ? 0 : 4);
     if (c < 4)
         c = 4;
@@ -1462,5 +1471,20 @@ RustError h2amd_g1_to_lagrange(void* d_out, const void* d_g, uint32_t k, const v
     H2_LAUNCH_CHECK();
     return ok();
 }
+
+// ------------------------------------ tuned window widths (no reference counterpart)
+
+RustError h2amd_msm_set_widths(const uint8_t* widths, uint32_t count)
+{
+    H2_CHECK_ARG(count <= 32, "h2amd_msm_set_widths: %u entries exceed 32", count);
+    for (uint32_t i = 0; i < count; ++i)
+        H2_CHECK_ARG(widths[i] == 0 || (widths[i] >= 4 && widths[i] <= 20),
+            "h2amd_msm_set_widths: width %u for 2^%u outside [4, 20]", widths[i], i);
+    for (uint32_t i = 0; i < 32; ++i)
+        g_width_table[i] = i < count ? widths[i] : 0;
+    return ok();
+}
+
+uint32_t h2amd_msm_width(uint64_t n) { return plan_width(n, false); }
 
 } // extern "C"

@@ -391,4 +391,6 @@ RustError h2amd_fft_batch_to_device(uint32_t ntt_type, uint32_t count, const uin
     return ntt_run_batch(jobs.data(), (int)count, tw, post, d_pass, stream);
 }
 
+RustError h2amd_ntt_set_tiles(const uint8_t* tiles, uint32_t count) { return ntt_set_tiles(tiles, count); }
+
 } // extern "C"

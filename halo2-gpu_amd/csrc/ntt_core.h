@@ -76,6 +76,10 @@ RustError ntt_run(const NttTables& t, uint32_t log_n, const NttJob* jobs, int co
 
 bool ntt_needs_scratch(uint32_t log_n, bool aliased);
 
+// Replaces the tuned tile table: tiles[log_n] = log2 of the elements per
+// work-group (0 = default rule, else 6..12) for count <= NTT_MAX_LOG + 1 sizes.
+RustError ntt_set_tiles(const uint8_t* tiles, uint32_t count);
+
 // One transform of an uneven batch: its own size and tables (shared between
 // jobs of one size), `in_len` valid inputs (the rest read as zero), in != out
 // when the transform takes two passes, and a scratch buffer of 2^log_n

@@ -79,6 +79,10 @@ static void host_to261(const Fr& x, uint32_t (&out)[9])
     out[8] = w[7] >> 8;
 }
 
+// Tuned work-group tile sizes: log2 of the elements per work-group for
+// transforms of 2^log_n (0 = the default rule). Set with h2amd_ntt_set_tiles.
+static uint8_t g_tile_table[NTT_MAX_LOG + 1] = { 0 };
+
 NttPlan ntt_plan(uint32_t log_n)
 {
     NttPlan p {};
@@ -103,6 +107,15 @@ NttPlan ntt_plan(uint32_t log_n)
             g = gmax;
         if (g < 0)
             g = 0;
+        if (log_n <= NTT_MAX_LOG && g_tile_table[log_n]) {
+            g = (int)g_tile_table[log_n] - (int)p.log_r[i];
+            if (g > gmax)
+                g = gmax;
+            if (g > (int)log_n - (int)p.log_r[i])
+                g = (int)log_n - (int)p.log_r[i];
+            if (g < 0)
+                g = 0;
+        }
         if (p.P == 1)
             g = 0;
         p.log_g[i] = (uint32_t)g;
@@ -585,6 +598,17 @@ RustError ntt_run_batch(const NttBatchJob* jobs, int count, NttTwist twist, NttP
             nlaunch[pass]);
         H2_LAUNCH_CHECK();
     }
+    return ok();
+}
+
+RustError ntt_set_tiles(const uint8_t* tiles, uint32_t count)
+{
+    H2_CHECK_ARG(count <= NTT_MAX_LOG + 1, "h2amd_ntt_set_tiles: %u entries exceed %u", count, NTT_MAX_LOG + 1);
+    for (uint32_t i = 0; i < count; ++i)
+        H2_CHECK_ARG(tiles[i] == 0 || (tiles[i] >= 6 && tiles[i] <= NTT_MAX_LOG_R),
+            "h2amd_ntt_set_tiles: tile 2^%u for 2^%u outside [2^6, 2^%u]", tiles[i], i, NTT_MAX_LOG_R);
+    for (uint32_t i = 0; i <= NTT_MAX_LOG; ++i)
+        g_tile_table[i] = i < count ? tiles[i] : 0;
     return ok();
 }
 
