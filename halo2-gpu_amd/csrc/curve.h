@@ -178,7 +178,7 @@ struct G1Xyzz {
     {
         if (is_inf())
             return G1Affine { Fq::zero(), Fq::zero() };
-        const Fq t = Fq::inverse(ZZ * ZZZ); // 1 / (ZZ ZZZ)
+        const Fq t = Fq::inverse_vt(ZZ * ZZZ); // 1 / (ZZ ZZZ)
         return G1Affine { X * (t * ZZZ), Y * (t * ZZ) };
     }
 };

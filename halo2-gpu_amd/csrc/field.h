@@ -28,6 +28,8 @@ struct FrParams {
         0x7879462eu, 0x666ea36fu, 0x9a07df2fu, 0x0e0a77c1u };
     static constexpr uint32_t R2[8] = { 0xae216da7u, 0x1bb8e645u, 0xe35c59e3u, 0x53fe3ab1u,
         0x53bb8085u, 0x8c49833du, 0x7f4e44a5u, 0x0216d0b1u };
+    static constexpr uint32_t R3[8] = { 0xb4bf0040u, 0x5e94d8e1u, 0x1cfbb6b8u, 0x2a489cbeu, 0xa19fcfedu,
+        0x893cc664u, 0x7fcc657cu, 0x0cf8594bu }; // 2^768 mod m
 };
 
 struct FqParams {
@@ -38,6 +40,8 @@ struct FqParams {
         0x7879462cu, 0x666ea36fu, 0x9a07df2fu, 0x0e0a77c1u };
     static constexpr uint32_t R2[8] = { 0x538afa89u, 0xf32cfc5bu, 0xd44501fbu, 0xb5e71911u,
         0x0a417ff6u, 0x47ab1effu, 0xcab8351fu, 0x06d89f71u };
+    static constexpr uint32_t R3[8] = { 0xda1530dfu, 0xb1cd6dafu, 0xa7283db6u, 0x62f210e6u, 0x0ada0afbu,
+        0xef7f0b0cu, 0x2d592544u, 0x20fd6e90u }; // 2^768 mod m
 };
 
 // acc(64) + x*y -> acc, carry-out of the 64-bit sum added into c2.
@@ -48,6 +52,80 @@ __device__ __forceinline__ void mac96(uint64_t& acc, uint32_t& c2, uint32_t x, u
         "v_addc_co_u32_e64 %2, %1, %2, 0, %1"
         : "+v"(acc), "=&s"(cm), "+v"(c2)
         : "v"(x), "v"(y));
+}
+
+// x^-1 mod m for 0 < x < m < 2^254, m odd (4 little-endian 64-bit limbs), by the
+// binary extended Euclidean algorithm: u = x, v = m, x1 = 1, x2 = 0 with
+// x1 x = u and x2 x = v (mod m); halve the even one of u, v (and its x mod m),
+// subtract the smaller from the larger, until u or v is 1. Variable time (the
+// data is public) and a chain of a few thousand 64-bit operations, against the
+// ~380 dependent Montgomery products of an exponentiation.
+__device__ inline void inv_mod_binary(const uint64_t (&x)[4], const uint64_t (&m)[4], uint64_t (&out)[4])
+{
+    uint64_t u[4] = { x[0], x[1], x[2], x[3] }, v[4] = { m[0], m[1], m[2], m[3] };
+    uint64_t x1[4] = { 1, 0, 0, 0 }, x2[4] = { 0, 0, 0, 0 };
+    auto is_one = [](const uint64_t (&a)[4]) { return a[0] == 1 && (a[1] | a[2] | a[3]) == 0; };
+    auto add = [](uint64_t (&a)[4], const uint64_t (&b)[4]) { // no carry out: a + b < 2^256
+        uint64_t c = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint64_t s = a[i] + b[i];
+            const uint64_t t = s + c;
+            c = (uint64_t)(s < a[i]) + (uint64_t)(t < s);
+            a[i] = t;
+        }
+    };
+    auto sub = [](uint64_t (&a)[4], const uint64_t (&b)[4]) -> bool { // a -= b, returns the borrow
+        uint64_t c = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint64_t d = a[i] - b[i];
+            const uint64_t t = d - c;
+            c = (uint64_t)(a[i] < b[i]) + (uint64_t)(d < c);
+            a[i] = t;
+        }
+        return c != 0;
+    };
+    auto geq = [](const uint64_t (&a)[4], const uint64_t (&b)[4]) {
+        for (int i = 3; i >= 0; --i)
+            if (a[i] != b[i])
+                return a[i] > b[i];
+        return true;
+    };
+    auto shr = [](uint64_t (&a)[4], int k) { // 0 < k < 64
+        a[0] = (a[0] >> k) | (a[1] << (64 - k));
+        a[1] = (a[1] >> k) | (a[2] << (64 - k));
+        a[2] = (a[2] >> k) | (a[3] << (64 - k));
+        a[3] >>= k;
+    };
+    auto halve = [&](uint64_t (&a)[4], uint64_t (&t)[4]) { // a >>= k (a even), t = t / 2^k mod m
+        const int k = a[0] ? __builtin_ctzll(a[0]) : 63;
+        shr(a, k);
+        for (int j = 0; j < k; ++j) {
+            if (t[0] & 1)
+                add(t, m); // t + m < 2m < 2^255
+            shr(t, 1);
+        }
+    };
+    while (!is_one(u) && !is_one(v)) {
+        while (!(u[0] & 1))
+            halve(u, x1);
+        while (!(v[0] & 1))
+            halve(v, x2);
+        if (geq(u, v)) {
+            sub(u, v);
+            if (sub(x1, x2))
+                add(x1, m);
+        } else {
+            sub(v, u);
+            if (sub(x2, x1))
+                add(x2, m);
+        }
+    }
+    const uint64_t (&r)[4] = is_one(u) ? x1 : x2;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        out[i] = r[i];
 }
 
 template <class P>
@@ -218,6 +296,32 @@ struct Fp256 {
                 r = mul(r, a);
         }
         return r;
+    }
+
+    // a^-1 by the binary extended Euclidean algorithm: plain inverse of the
+    // stored word A = a R, times R^3 with one Montgomery product (A^-1 R^2 = a^-1 R);
+    // inverse_vt(0) = 0
+    __device__ static Fp256 inverse_vt(const Fp256& a)
+    {
+        if (a.is_zero())
+            return zero();
+        uint64_t x[4], m[4], r[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            x[i] = ((uint64_t)a.v[2 * i + 1] << 32) | a.v[2 * i];
+            m[i] = ((uint64_t)P::M[2 * i + 1] << 32) | P::M[2 * i];
+        }
+        inv_mod_binary(x, m, r);
+        Fp256 ri, r3;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            ri.v[2 * i] = (uint32_t)r[i];
+            ri.v[2 * i + 1] = (uint32_t)(r[i] >> 32);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            r3.v[i] = P::R3[i];
+        return mul(ri, r3);
     }
 
     // a^(M-2) with 4-bit fixed windows; inverse(0) = 0

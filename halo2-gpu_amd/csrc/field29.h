@@ -42,6 +42,9 @@ struct Fr29Params {
     // 2^261 mod m as a 256-bit integer: Fp256::mul(x * 2^256, TO261) = x * 2^261
     static constexpr uint32_t TO261[8] = { 0x8fffff57u, 0x2fd4e156u, 0xa494b01au, 0x75bba827u, 0x819caa80u,
         0x5301fa84u, 0x563d4475u, 0x0dc83629u };
+    // 2^783 mod m in 29-bit limbs: mul(A^-1, C783) = A^-1 * 2^522
+    static constexpr uint32_t C783[9] = { 0x001fddb2u, 0x17d30b63u, 0x1a2600eeu, 0x09507c47u, 0x1496b29bu,
+        0x0b00a268u, 0x15b645ebu, 0x1f9fcb3du, 0x001baa96u };
 };
 
 struct Fq29Params {
@@ -59,6 +62,8 @@ struct Fq29Params {
     // 2^256 mod m in 29-bit limbs: mul(x * 2^261, FROM261) = x * 2^256
     static constexpr uint32_t FROM261[9] = { 0x058f0d9du, 0x1aea1c6eu, 0x11c2cf74u, 0x11d651ebu, 0x1462c0a7u,
         0x11b7bc3cu, 0x1cbd99bau, 0x183340fbu, 0x000e0a77u };
+    static constexpr uint32_t C783[9] = { 0x0e2312b2u, 0x16c05ca2u, 0x0bc84389u, 0x1cdf310bu, 0x11adafddu,
+        0x032e568eu, 0x1d6ae48cu, 0x10d4cd1fu, 0x0026c2d2u };
 };
 
 template <class P>
@@ -320,6 +325,33 @@ struct F29 {
             }
         }
         return r;
+    }
+
+    // The same value as pow_m_minus_2 (u^-1 * 2^261 for a = u * 2^261, i.e.
+    // 2^522 / A for the integer A = a mod m; zero maps to zero) by the binary
+    // extended Euclidean algorithm and one product with 2^783. Input normalised,
+    // < 2m; output a normalised representative < 1.01m.
+    __device__ static F29 inverse_vt(const F29 a)
+    {
+        const F29 c = reduce_full(a);
+        if (c.is_zero())
+            return zero();
+        uint32_t w[8];
+        c.to32(w);
+        uint64_t x[4], m[4], r[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            x[i] = ((uint64_t)w[2 * i + 1] << 32) | w[2 * i];
+            m[i] = ((uint64_t)P::Base::M[2 * i + 1] << 32) | P::Base::M[2 * i];
+        }
+        inv_mod_binary(x, m, r);
+        uint32_t rw[8];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            rw[2 * i] = (uint32_t)r[i];
+            rw[2 * i + 1] = (uint32_t)(r[i] >> 32);
+        }
+        return mul(from32(rw), from_limbs(P::C783));
     }
 
     // normalised input of value < 2^261 -> canonical representative in [0, m)

@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(T) batch_invert_kernel(uint8_t* __restrict__ d
     if (tid == 0) {
         F k251 = F::zero();
         k251.l[8] = 1u << 19; // 2^251 = 2^(232 + 19)
-        inv_total = F::mul(F::pow_m_minus_2(pre[T - 1]), k251);
+        inv_total = F::mul(F::inverse_vt(pre[T - 1]), k251);
     }
     __syncthreads();
     // inverse of this thread's product = (prod of others before) * (after) / total
@@ -103,13 +103,13 @@ __global__ void __launch_bounds__(T) batch_invert_kernel(uint8_t* __restrict__ d
     }
 }
 
-// ------------------------------------- with a workspace: few exponentiations
+// ------------------------------------- with a workspace: few inversions
 //
-// With more than one work-group, the per-group exponentiation above (a chain of
-// ~380 dependent products on one lane) dominates. Callers that hold a
+// With more than one work-group, the per-group inversion above (a dependent chain
+// on one lane) dominates. Callers that hold a
 // workspace take three steps instead: (1) every group writes the product of its
 // elements to tot[g]; (2) the totals are inverted by the kernel above (one
-// exponentiation per 2048 totals, in parallel groups); (3) every group finishes
+// inversion per 2048 totals, in parallel groups); (3) every group finishes
 // with its inverted total. The values match the single-kernel path bit for bit.
 
 // (1) tot[g] = product of the non-zero elements of group g (2^261-scale word)
@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(T) group_finish_kernel(uint8_t* __restrict__ d
 // slot s (s < 8) of group j is the 256 bits held by the 128 elements
 // [jC + 128 s, jC + 128 (s + 1)) and carries the total of group 8j + s, then its
 // inverse. Readers mask the spare bits; every spare bit is written by one thread.
-// The middle step inverts the totals in blocks of kMidPer (one exponentiation per
+// The middle step inverts the totals in blocks of kMidPer (one inversion per
 // block, the blocks in parallel); the finishing groups run host-last (group j
 // hosts 8j .. 8j + 7), so no group overwrites a slot another still has to read.
 constexpr unsigned kSlot = 128; // elements per slot, 2 bits each
@@ -325,7 +325,7 @@ __global__ void __launch_bounds__(kMidThreads) spare_inverse_kernel(uint8_t* __r
     if (tid == 0) {
         F k251 = F::zero();
         k251.l[8] = 1u << 19; // 2^251 = 2^(232 + 19)
-        inv_all = F::mul(F::pow_m_minus_2(pre[kMidThreads - 1]), k251);
+        inv_all = F::mul(F::inverse_vt(pre[kMidThreads - 1]), k251);
     }
     __syncthreads();
     F inv = inv_all;
