@@ -74,6 +74,8 @@ struct PointSlot {
 struct MsmPlan {
     uint64_t n; // points (for a batch: the virtual range of all jobs)
     uint64_t nbase; // bases converted to the 2^261 scale (a batch shares one base array)
+    bool glv; // scalars split k = k1 + lambda k2: 2 nbase virtual points of 128-bit scalars
+    uint64_t pts_count; // bases held in the 2^261 scale: nbase, or 2 nbase with the endomorphism images
     uint32_t J; // MSMs in the pipeline (1 unless batched); each has its own W bucket sets
     uint32_t c, W, B;
     uint32_t Wb; // bucket windows: W, or 1 when the windows share buckets (precomputed bases)
@@ -107,6 +109,19 @@ uint32_t log2_floor(uint64_t n)
 // the default rule). Set with h2amd_msm_set_widths before sizing and launching.
 uint8_t g_width_table[32] = { 0 };
 
+// n (virtual) points of `bits`-bit signed-digit scalars
+uint32_t default_width(uint64_t n, bool shared, uint32_t bits)
+{
+    int c = (int)log2_floor(n < 1 ? 1 : n) - (shared ? 0 : 4);
+    if (c < 4)
+        c = 4;
+    if (c > (shared ? 22 : 20))
+        c = shared ? 22 : 20;
+    // same window count, narrowest width: the top window is as full as the others
+    const uint32_t w0 = (bits + (uint32_t)c - 1) / (uint32_t)c;
+    return (bits + w0 - 1) / w0;
+}
+
 uint32_t plan_width(uint64_t n, bool shared)
 {
     if (!shared) {
@@ -114,27 +129,40 @@ uint32_t plan_width(uint64_t n, bool shared)
         if (tuned)
             return tuned;
     }
-    int c = (int)log2_floor(n < 1 ? 1 : n) - (shared ? 0 : 4);
-    if (c < 4)
-        c = 4;
-    if (c > (shared ? 22 : 20))
-        c = shared ? 22 : 20;
-    // same window count, narrowest width: the top window is as full as the others
-    const uint32_t w0 = (255 + (uint32_t)c - 1) / (uint32_t)c;
-    return (255 + w0 - 1) / w0;
+    return default_width(n, shared, 255);
+}
+
+// GLV: 2n virtual points with scalars below 2^128 (signed digits need 129 bits).
+// It halves the windows, so the per-window reduction and the Horner over
+// windows (a serial chain of ~254 doublings without it): the win below 2^21
+// points, where they dominate. Above, the additions are the cost and they do
+// not shrink (2n points x 129/c windows), while the sort and bucket traffic
+// grow by the rounding of the window count, so plain scalars stay.
+constexpr uint32_t GLV_BITS = 129;
+constexpr uint64_t kGlvBelow = 1ull << 21;
+
+bool use_glv(uint64_t npoints) { return npoints < kGlvBelow; }
+
+uint32_t plain_msm_width(uint64_t npoints)
+{
+    const uint32_t tuned = g_width_table[log2_floor(npoints < 1 ? 1 : npoints) & 31];
+    if (tuned)
+        return tuned;
+    return use_glv(npoints) ? default_width(2 * npoints, false, GLV_BITS) : default_width(npoints, false, 255);
 }
 
 // n points (virtual range), window width c, J MSMs; table_n > 0 selects
 // precomputed bases (one MSM, shared buckets).
-MsmPlan plan_core(uint64_t n, uint32_t c, uint32_t J, uint64_t table_n)
+MsmPlan plan_core(uint64_t n, uint32_t c, uint32_t J, uint64_t table_n, uint32_t bits = 255)
 {
     MsmPlan p {};
     p.n = n;
     p.nbase = n;
+    p.pts_count = n;
     p.J = J;
     p.table_n = table_n;
     p.c = c;
-    p.W = (255 + p.c - 1) / p.c;
+    p.W = (bits + p.c - 1) / p.c;
     p.Wb = table_n ? 1u : J * p.W;
     p.B = 1u << (p.c - 1);
     p.nb = (uint64_t)p.Wb * p.B;
@@ -175,7 +203,15 @@ MsmPlan plan_core(uint64_t n, uint32_t c, uint32_t J, uint64_t table_n)
 // table_n points (the window width follows the table).
 MsmPlan make_plan(uint64_t n, uint64_t table_n = 0)
 {
-    return plan_core(n, plan_width(table_n ? table_n : n, table_n != 0), 1, table_n);
+    if (table_n)
+        return plan_core(n, plan_width(table_n, true), 1, table_n);
+    if (!use_glv(n))
+        return plan_core(n, plain_msm_width(n), 1, 0);
+    MsmPlan p = plan_core(2 * n, plain_msm_width(n), 1, 0, GLV_BITS);
+    p.glv = true;
+    p.nbase = n;
+    p.pts_count = 2 * n;
+    return p;
 }
 
 uint64_t align_tile(uint64_t n) { return (n + R1_TILE - 1) / R1_TILE * R1_TILE; }
@@ -198,6 +234,7 @@ MsmPlan make_batch_plan(const uint64_t* lengths, uint32_t J)
     p.n = nv;
     p.ntiles = (uint32_t)(nv / R1_TILE);
     p.nbase = nmax;
+    p.pts_count = nmax;
     return p;
 }
 
@@ -234,7 +271,7 @@ void carve(WS& ws, const MsmPlan& p, MsmBuffers* b)
     const uint64_t nred = (uint64_t)p.Wb * p.Mlev[0];
     void* v[24];
     int k = 0;
-    v[k++] = ws.take(p.table_n ? 0 : 64 * p.nbase);
+    v[k++] = ws.take(p.table_n ? 0 : 64 * p.pts_count);
     v[k++] = ws.take(p.J > 1 ? sizeof(BatchJob) * p.J : 0);
     v[k++] = ws.take(p.J > 1 ? 4ull * p.ntiles : 0);
     v[k++] = ws.take(32 * p.n);
@@ -320,6 +357,131 @@ __global__ void to_scale261_kernel(const uint8_t* __restrict__ in, uint8_t* __re
         k.v[q] = Fq29Params::TO261[q];
     (Fq::load(in + 64 * i) * k).store(out + 64 * i);
     (Fq::load(in + 64 * i + 32) * k).store(out + 64 * i + 32);
+}
+
+// ------------------------------------------------------- GLV split
+//
+// BN254 G1 has the endomorphism phi(x, y) = (beta x, y) = lambda (x, y), with
+// lambda and beta primitive cube roots of unity mod r and p (lambda is the
+// coset generator zeta). A scalar k = k1 + lambda k2 (mod r) with |k1|, |k2| <
+// 2^128 turns k P into k1 P + k2 phi(P): 2n points of 128-bit scalars, half the
+// windows. The split is Babai rounding against the lattice basis
+// (a1, -a2), (a2, b2) of {(x, y) : x + lambda y = 0 mod r}, found by the
+// extended Euclidean algorithm on (r, lambda): c1 = floor(k g1 / 2^256),
+// c2 = floor(k g2 / 2^256) with g1 = floor(b2 2^256 / r), g2 = floor(a2 2^256 / r),
+// then k1 = k - c1 a1 - c2 a2 and k2 = c1 a2 - c2 b2. Any c1, c2 give a valid
+// split; the rounding keeps |k1|, |k2| below 2^128 (below 2^127 in practice).
+constexpr uint32_t GLV_G1[5] = { 0x00ff6565u, 0x5398fd03u, 0xa773d2d2u, 0x4ccef014u, 0x00000002u };
+constexpr uint32_t GLV_G2[3] = { 0xc7e0b3d7u, 0xd91d232eu, 0x00000002u };
+constexpr uint32_t GLV_A1[4] = { 0x7d4f1128u, 0x8211bbebu, 0xeeb859fcu, 0x6f4d8248u };
+constexpr uint32_t GLV_A2[2] = { 0x94d213e3u, 0x89d32568u };
+constexpr uint32_t GLV_B2[4] = { 0x1221250bu, 0x0be4e154u, 0xeeb859fdu, 0x6f4d8248u };
+// beta * 2^256 mod p (Montgomery form)
+constexpr uint32_t GLV_BETA[8] = { 0x13e80b9cu, 0x3350c88eu, 0xdb5e56b9u, 0x7dce557cu, 0xb615564au, 0x6001b4b8u,
+    0x020217e0u, 0x2682e617u };
+
+// out[0, na + nb) = a * b (schoolbook over 32-bit limbs)
+template <int NA, int NB>
+__device__ __forceinline__ void mul_words(const uint32_t (&a)[NA], const uint32_t (&b)[NB], uint32_t (&out)[NA + NB])
+{
+#pragma unroll
+    for (int i = 0; i < NA + NB; ++i)
+        out[i] = 0;
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+        uint64_t carry = 0;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const uint64_t t = (uint64_t)a[i] * b[j] + out[i + j] + carry;
+            out[i + j] = (uint32_t)t;
+            carry = t >> 32;
+        }
+        out[i + NB] = (uint32_t)carry;
+    }
+}
+
+// acc (N words, two's complement) -= x (M <= N words)
+template <int N, int M>
+__device__ __forceinline__ void sub_words(uint32_t (&acc)[N], const uint32_t (&x)[M])
+{
+    uint32_t borrow = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const uint64_t d = (uint64_t)acc[i] - (i < M ? x[i] : 0u) - borrow;
+        acc[i] = (uint32_t)d;
+        borrow = (uint32_t)(d >> 32) & 1u;
+    }
+}
+
+// |v| of an N-word two's complement value into the low 4 words of out (word 7 bit 31 = sign)
+template <int N>
+__device__ __forceinline__ void store_half(const uint32_t (&v)[N], uint32_t* __restrict__ out)
+{
+    const bool neg = (v[N - 1] >> 31) != 0;
+    uint32_t m[4];
+    uint32_t carry = 1;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (neg) {
+            const uint64_t t = (uint64_t)(~v[i]) + carry;
+            m[i] = (uint32_t)t;
+            carry = (uint32_t)(t >> 32);
+        } else {
+            m[i] = v[i];
+        }
+    }
+    uint4* q = (uint4*)out;
+    q[0] = make_uint4(m[0], m[1], m[2], m[3]);
+    q[1] = make_uint4(0u, 0u, 0u, neg ? 0x80000000u : 0u);
+}
+
+// canon[2i] = (|k1|, sign), canon[2i + 1] = (|k2|, sign) for k = canonical(scal[i])
+__global__ void glv_split_kernel(const Fr* __restrict__ scal, uint32_t* __restrict__ canon, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const Fr kv = Fr::from_mont(Fr::load(scal + i));
+    uint32_t k[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        k[q] = kv.v[q];
+    uint32_t p1[13], p2[11];
+    mul_words<8, 5>(k, GLV_G1, p1);
+    mul_words<8, 3>(k, GLV_G2, p2);
+    const uint32_t c1[4] = { p1[8], p1[9], p1[10], p1[11] }; // k g1 < 2^384: c1 < 2^128
+    const uint32_t c2[3] = { p2[8], p2[9], p2[10] };
+    uint32_t m1[8], m2[5], m3[6], m4[7];
+    mul_words<4, 4>(c1, GLV_A1, m1);
+    mul_words<3, 2>(c2, GLV_A2, m2);
+    mul_words<4, 2>(c1, GLV_A2, m3);
+    mul_words<3, 4>(c2, GLV_B2, m4);
+    uint32_t k1[9] = { k[0], k[1], k[2], k[3], k[4], k[5], k[6], k[7], 0u };
+    sub_words<9, 8>(k1, m1);
+    sub_words<9, 5>(k1, m2);
+    uint32_t k2[8] = { m3[0], m3[1], m3[2], m3[3], m3[4], m3[5], 0u, 0u };
+    sub_words<8, 7>(k2, m4);
+    store_half<9>(k1, canon + 16 * i);
+    store_half<8>(k2, canon + 16 * i + 8);
+}
+
+// pts[2i] = g_i, pts[2i + 1] = phi(g_i) = (beta x, y), both in the 2^261 scale; (0, 0) stays (0, 0)
+__global__ void glv_bases_kernel(const uint8_t* __restrict__ in, uint8_t* __restrict__ out, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    Fq k, beta;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        k.v[q] = Fq29Params::TO261[q];
+        beta.v[q] = GLV_BETA[q];
+    }
+    const Fq x = Fq::load(in + 64 * i) * k, y = Fq::load(in + 64 * i + 32) * k;
+    x.store(out + 128 * i);
+    y.store(out + 128 * i + 32);
+    (x * beta).store(out + 128 * i + 64);
+    y.store(out + 128 * i + 96);
 }
 
 // signed digit of window w; `carry` flows from window w-1 to w
@@ -461,7 +623,9 @@ __global__ void __launch_bounds__(R1_THREADS) r1_scatter_kernel(const uint32_t* 
                 const uint32_t bi = (uint32_t)(d < 0 ? -d : d) - 1u;
                 bin[k] = bi >> sp.FB;
                 const uint32_t idx = sp.table_n ? (uint32_t)(w * sp.table_n + i) : (uint32_t)(i - tr.v0);
-                ent[k] = ((uint64_t)(bi & fmask) << 32) | idx | (d < 0 ? 0x80000000u : 0u);
+                // the point's own sign (a negative GLV half) rides in bit 31 of scalar word 7
+                const bool neg = (d < 0) != ((s[k][7] >> 31) != 0);
+                ent[k] = ((uint64_t)(bi & fmask) << 32) | idx | (neg ? 0x80000000u : 0u);
                 rank[k] = atomicAdd(&base[bin[k]], 1u);
             }
         }
@@ -1108,12 +1272,19 @@ RustError msm_device(const MsmPlan& p, const MsmBuffers& b, const Fr* d_scalars,
     const unsigned T = 256;
     ProfScope prof_all("msm_total", stream);
     std::unique_ptr<ProfScope> prof_sort(new ProfScope("msm_sort", stream));
-    if (d_scalars) {
+    if (d_scalars && p.glv) {
+        glv_split_kernel<<<blocks_for(p.nbase, T), T, 0, stream>>>(d_scalars, b.canon, p.nbase);
+        H2_LAUNCH_CHECK();
+    } else if (d_scalars) {
         to_canonical_kernel<<<blocks_for(p.n, T), T, 0, stream>>>(d_scalars, b.canon, p.n);
         H2_LAUNCH_CHECK();
     }
     const uint8_t* pts = d_points; // a precomputed table is already in the 2^261 scale
-    if (!p.table_n) {
+    if (p.glv) {
+        glv_bases_kernel<<<blocks_for(p.nbase, T), T, 0, stream>>>(d_points, b.pts, p.nbase);
+        H2_LAUNCH_CHECK();
+        pts = b.pts;
+    } else if (!p.table_n) {
         to_scale261_kernel<<<blocks_for(p.nbase, T), T, 0, stream>>>(d_points, b.pts, p.nbase);
         H2_LAUNCH_CHECK();
         pts = b.pts;
@@ -1182,7 +1353,8 @@ RustError msm_entry(const char* who, const void* scalars, bool scalars_on_device
         memcpy(o + 4, one.v, 32);
         return ok();
     }
-    H2_CHECK_ARG(n < (1ull << 31), "%s: length %lu exceeds 2^31", who, (unsigned long)n);
+    // two virtual points per base (GLV): indices stay below 2^31 (bit 31 is the sign)
+    H2_CHECK_ARG(n < (1ull << 30), "%s: length %lu exceeds 2^30", who, (unsigned long)n);
     const MsmPlan p = make_plan(n);
     H2_CHECK_ARG((uint64_t)p.n * p.W < (1ull << 32), "%s: length %lu too large for 32-bit bucket offsets", who,
         (unsigned long)n);
@@ -1485,6 +1657,6 @@ RustError h2amd_msm_set_widths(const uint8_t* widths, uint32_t count)
     return ok();
 }
 
-uint32_t h2amd_msm_width(uint64_t n) { return plan_width(n, false); }
+uint32_t h2amd_msm_width(uint64_t n) { return plain_msm_width(n); }
 
 } // extern "C"
