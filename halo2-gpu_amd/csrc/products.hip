@@ -200,7 +200,8 @@ __global__ void __launch_bounds__(T) group_finish_kernel(uint8_t* __restrict__ d
 // hosts 8j .. 8j + 7), so no group overwrites a slot another still has to read.
 constexpr unsigned kSlot = 128; // elements per slot, 2 bits each
 constexpr unsigned kMidThreads = 64;
-constexpr unsigned kMidPer = kMidThreads * E; // totals per middle block
+constexpr unsigned kMidE = 2; // totals per middle thread: short chains, many blocks
+constexpr unsigned kMidPer = kMidThreads * kMidE; // totals per middle block
 
 template <class P>
 __device__ __forceinline__ F29<P> load_masked(const uint8_t* p)
@@ -287,25 +288,49 @@ __global__ void __launch_bounds__(T) spare_total_kernel(uint8_t* __restrict__ da
     }
 }
 
+// The totals of the middle step: in the elements' spare bits, or in a workspace array.
+struct SpareTotals {
+    uint8_t* data;
+    __device__ void get(uint64_t g, uint32_t (&w)[8]) const { slot_get(data, slot_base(g), w); }
+    __device__ void put(uint64_t g, const uint32_t (&w)[8]) const { slot_put(data, slot_base(g), w); }
+};
+struct PlainTotals {
+    uint8_t* tot;
+    __device__ void get(uint64_t g, uint32_t (&w)[8]) const
+    {
+        const uint4* q = (const uint4*)(tot + 32 * g);
+        const uint4 a = q[0], b = q[1];
+        w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w;
+        w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
+    }
+    __device__ void put(uint64_t g, const uint32_t (&w)[8]) const
+    {
+        uint4* q = (uint4*)(tot + 32 * g);
+        q[0] = make_uint4(w[0], w[1], w[2], w[3]);
+        q[1] = make_uint4(w[4], w[5], w[6], w[7]);
+    }
+};
+
 // (2) block b inverts totals [b kMidPer, (b + 1) kMidPer): each becomes t^-1 * 2^-10
-// in the 2^261 scale, the single-group kernel's inv_total
-template <class P>
-__global__ void __launch_bounds__(kMidThreads) spare_inverse_kernel(uint8_t* __restrict__ data, uint64_t groups)
+// in the 2^261 scale, the single-group kernel's inv_total (the same value the
+// batch inversion of the totals array gives)
+template <class P, class A>
+__global__ void __launch_bounds__(kMidThreads) totals_inverse_kernel(A acc_totals, uint64_t groups)
 {
     using F = F29<P>;
     __shared__ F pre[kMidThreads];
     __shared__ F suf[kMidThreads];
     __shared__ F inv_all;
     const unsigned tid = threadIdx.x;
-    const uint64_t g0 = (uint64_t)blockIdx.x * kMidPer + (uint64_t)tid * E;
-    F part[E];
+    const uint64_t g0 = (uint64_t)blockIdx.x * kMidPer + (uint64_t)tid * kMidE;
+    F part[kMidE];
     F acc = F::one();
 #pragma unroll
-    for (unsigned k = 0; k < E; ++k) {
+    for (unsigned k = 0; k < kMidE; ++k) {
         part[k] = acc;
         if (g0 + k < groups) {
             uint32_t w[8];
-            slot_get(data, slot_base(g0 + k), w);
+            acc_totals.get(g0 + k, w);
             acc = F::mul(acc, F::from32(w)); // a total is never zero
         }
     }
@@ -334,13 +359,13 @@ __global__ void __launch_bounds__(kMidThreads) spare_inverse_kernel(uint8_t* __r
     if (tid + 1 < kMidThreads)
         inv = F::mul(inv, suf[tid + 1]);
 #pragma unroll
-    for (int k = (int)E - 1; k >= 0; --k) {
+    for (int k = (int)kMidE - 1; k >= 0; --k) {
         if (g0 + k < groups) {
             uint32_t w[8], o[8];
-            slot_get(data, slot_base(g0 + k), w);
+            acc_totals.get(g0 + k, w);
             F::reduce_full(F::mul(inv, part[k])).to32(o);
             inv = F::mul(inv, F::from32(w));
-            slot_put(data, slot_base(g0 + k), o);
+            acc_totals.put(g0 + k, o);
         }
     }
 }
@@ -439,7 +464,8 @@ RustError batch_invert(F* data, uint64_t n, hipStream_t stream, void* ws = nullp
         uint8_t* d = (uint8_t*)data;
         spare_total_kernel<P><<<(unsigned)groups, T, 0, stream>>>(d, n);
         H2_LAUNCH_CHECK();
-        spare_inverse_kernel<P><<<(unsigned)((groups + kMidPer - 1) / kMidPer), kMidThreads, 0, stream>>>(d, groups);
+        totals_inverse_kernel<P><<<(unsigned)((groups + kMidPer - 1) / kMidPer), kMidThreads, 0, stream>>>(
+            SpareTotals { d }, groups);
         H2_LAUNCH_CHECK();
         // hosts after the groups they host: [ceil(g/8), g), then [ceil(g/64), ceil(g/8)), ...
         for (uint64_t hi = groups; hi > 0;) {
@@ -453,7 +479,8 @@ RustError batch_invert(F* data, uint64_t n, hipStream_t stream, void* ws = nullp
     uint8_t* tot = (uint8_t*)ws;
     group_total_kernel<P><<<(unsigned)groups, T, 0, stream>>>((const uint8_t*)data, n, tot);
     H2_LAUNCH_CHECK();
-    batch_invert_kernel<P><<<(unsigned)((groups + C - 1) / C), T, 0, stream>>>(tot, groups);
+    totals_inverse_kernel<P><<<(unsigned)((groups + kMidPer - 1) / kMidPer), kMidThreads, 0, stream>>>(
+        PlainTotals { tot }, groups);
     H2_LAUNCH_CHECK();
     group_finish_kernel<P><<<(unsigned)groups, T, 0, stream>>>((uint8_t*)data, n, tot);
     H2_LAUNCH_CHECK();
