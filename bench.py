@@ -47,6 +47,9 @@ def parse():
                     help="skip the per-size MSM width / NTT tile measurements against the default rules")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the uneven batch (32 commit MSMs + coeff_to_extended, per poly vs batched)")
+    ap.add_argument("--proof-k", type=int, default=20, help="k of the standard-gate proof measured under extra")
+    ap.add_argument("--proof-reps", type=int, default=10)
+    ap.add_argument("--no-proof", action="store_true", help="skip the end-to-end proof measurement")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one plain device copy of this size first (FETCH_SIZE/WRITE_SIZE calibration)")
     return ap.parse_args()
@@ -210,6 +213,37 @@ def batch_measurements(msm, torch, seed: int = 2024, count: int = 32) -> dict:
     return out
 
 
+def proof_measurements(msm, torch, k: int, reps: int) -> dict:
+    """End-to-end proofs of the standard-gate circuit at 2^k rows (halo2_amd.prover:
+    5 advice columns, copy cycles, a 16-bit range lookup, SHPLONK opening), one after
+    another on one GPU with the proving key and SRS resident: proofs/s, p50/p99 latency
+    and the per-phase split of the median proof. Every proof's self-checks must hold."""
+    import numpy as np
+
+    from halo2_amd import prover
+
+    circuit = prover.synthesize_standard(k, 5)
+    pk = prover.keygen(circuit)
+    params = msm.ParamsKZG(k, 0x5EED, fixed_base=False)
+    _ = params.g_lagrange_device
+    prover.create_proof(params, pk, circuit, seed=99)  # warm-up
+    lat, runs = [], []
+    ta = time.perf_counter()
+    for i in range(reps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = prover.create_proof(params, pk, circuit, seed=i + 1)
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - t) * 1e3)
+        runs.append(out)
+    wall = time.perf_counter() - ta
+    med = int(np.argsort(lat)[len(lat) // 2])
+    return {"k": k, "proofs": reps, "proofs_per_s": round(reps / wall, 3),
+            "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+            "phases_ms": runs[med]["times_ms"], "proof_bytes": len(runs[med]["proof"]),
+            "all_checks_hold": all(all(r["checks"].values()) for r in runs)}
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -362,6 +396,10 @@ def main() -> int:
     # ------------------------------- uneven batch (32 polys, k in 14..22)
     if not args.no_batch:
         extra["batch_uneven_32"] = batch_measurements(msm, torch)
+
+    # ------------------------- end-to-end proof of the standard-gate circuit
+    if not args.no_proof:
+        extra[f"proof_k{args.proof_k}"] = proof_measurements(msm, torch, args.proof_k, args.proof_reps)
 
     # ------------------ tuner: MSM width / NTT tile per size vs the default rules
     if not args.no_autotune:

@@ -877,6 +877,34 @@ __global__ void __launch_bounds__(R2_THREADS) r2_scatter_kernel(const uint64_t* 
         entries[cnt[skey[i]] + i] = sval[i];
 }
 
+// The nonempty bucket holding entry e, given off[lo] <= e (lo is the bucket
+// after the one that just ended at e): the largest x >= lo with off[x] <= e.
+// Galloping, then bisection: one step when bucket lo is nonempty (the common
+// case), O(log gap) over a run of empty buckets (small scalars leave whole
+// windows empty, hundreds of thousands of buckets in a row).
+__device__ __forceinline__ uint32_t next_bucket(const uint32_t* __restrict__ off, uint32_t lo, uint32_t nb, uint32_t e)
+{
+    uint32_t hi = nb; // off[nb] = E > e
+    for (uint32_t step = 1;; step <<= 1) {
+        const uint64_t probe = (uint64_t)lo + step;
+        if (probe >= nb)
+            break;
+        if (off[probe] > e) {
+            hi = (uint32_t)probe;
+            break;
+        }
+        lo = (uint32_t)probe;
+    }
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= e)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
 // Thread t adds the sorted entries [tS, min(tS + S, E)). A bucket wholly in
 // the range goes to bsum; the piece of a bucket that started before the
 // range goes to part[2t], the piece of one that runs past it to part[2t+1].
@@ -907,10 +935,8 @@ __global__ void __launch_bounds__(256) accumulate_kernel(const uint32_t* __restr
             acc.store(first ? (void*)(part + 2 * t) : (void*)(bsum + b));
             first = false;
             acc = G1X29::inf();
-            do {
-                ++b;
-                bend = off[b + 1];
-            } while (bend == e);
+            b = next_bucket(off, b + 1, (uint32_t)nb, e);
+            bend = off[b + 1];
         }
         const uint32_t xw[8] = { r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w };
         const uint32_t yw[8] = { r2.x, r2.y, r2.z, r2.w, r3.x, r3.y, r3.z, r3.w };

@@ -406,3 +406,28 @@ def test_g_to_lagrange_identities_and_downsize(lib, co):
     assert (dv.to_host(parts.g_lagrange_device, cols=8) == dv.to_host(small.g_lagrange_device, cols=8)).all()
     poly = rand_fr(co, 512, 3)
     assert (parts.commit_lagrange(poly) == small.commit_lagrange(poly)).all()
+
+
+def test_msm_small_scalars_empty_windows(lib, co):
+    """Lookup-style scalars (values below 2^16, sorted, most of a table column zero)
+    leave every window above the second empty: long runs of empty buckets between
+    filled ones, single and batched (where the empty runs span jobs), against the oracle."""
+    from halo2_amd import device as dv, fields as F, msm
+
+    n = 1 << 14
+    rng = np.random.default_rng(515)
+    small = np.sort(rng.integers(0, 1 << 16, n))
+    table = np.zeros(n, dtype=np.int64)
+    table[-(n // 10):] = np.arange(n // 10)
+    cols = [np.stack([F.mont_limbs(int(v)) for v in vals]) for vals in (small, table)]
+    ds = msm.random_fr(n, 516)
+    pts = msm.g1_mul_generator(ds, n)
+    hp = dv.to_host(pts, cols=8)
+    print("msm small scalars: single", flush=True)
+    for c in cols:
+        assert (msm.multiexp_device_scalars_device_bases(dv.to_device(c), pts, n) == co.msm(c, hp)).all()
+    print("msm small scalars: batch", flush=True)
+    jobs = [rand_fr(co, n, 517), cols[0], cols[1], rand_fr(co, n, 518), cols[0]]
+    got = msm.multiexp_batch_device([dv.to_device(c) for c in jobs], pts)
+    for j, c in enumerate(jobs):
+        assert (got[j] == co.msm(c, hp)).all(), f"job {j}"
