@@ -34,15 +34,36 @@ struct Sizer {
 
 // ------------------------------------------------------------ evaluation
 
-// A work-group covers kEvalT * kEvalM consecutive coefficients from `base`;
-// thread t folds a_{base + t + m kEvalT} over m by Horner in U = u^kEvalT
-// (coalesced: for each m the group reads kEvalT consecutive words), then
-// scales by u^(base + t).
+// A work-group covers kEvalSpan = kEvalT * kEvalRows consecutive coefficients
+// from base = blockIdx.x * kEvalSpan: thread t folds a_{base + t + m kEvalT}
+// over m by Horner in U = u^kEvalT (coalesced: for each m the group reads
+// kEvalT consecutive words) and scales by u^t; the group sum is
+// P_b = sum_i a_{base + i} u^i, without the factor u^base. The combine kernel
+// then adds sum_b P_b V^b, V = u^kEvalSpan, by the same scheme over blocks.
+// Powers u^0..u^kEvalT are built in LDS by doubling (8 dependent products),
+// so a thread's chain of dependent products is about 8 + kEvalRows + 1 per
+// kernel: evaluation is latency-bound at these sizes, not bandwidth-bound.
 constexpr unsigned kEvalT = 256;
-constexpr unsigned kEvalM = 64;
-constexpr uint64_t kEvalSpan = (uint64_t)kEvalT * kEvalM;
+constexpr unsigned kEvalRows = 16;
+constexpr unsigned kEvalLogRows = 4;
+constexpr uint64_t kEvalSpan = (uint64_t)kEvalT * kEvalRows;
+constexpr unsigned kEvalMaxJobs = 32; // polynomials per launch (the table is a kernel argument)
 
-uint64_t eval_blocks(uint64_t n) { return (n + kEvalSpan - 1) / kEvalSpan; }
+__host__ __device__ inline uint64_t eval_blocks(uint64_t n) { return (n + kEvalSpan - 1) / kEvalSpan; }
+
+// One polynomial: coefficients a[0, n), the point (from du when set, else u),
+// the result slot.
+struct EvalJob {
+    const Fr* a;
+    uint64_t n;
+    const Fr* du;
+    Fr u;
+    Fr* out;
+};
+
+struct EvalJobs {
+    EvalJob j[kEvalMaxJobs];
+};
 
 // bytes: point, per-group sums, result
 uint64_t eval_scratch(uint64_t n) { return 3 * 256 + align_up(32 * (eval_blocks(n) + 1), 256); }
@@ -75,59 +96,109 @@ __device__ __forceinline__ Fr block_sum(Fr v, Fr* sh)
     return sh[0];
 }
 
-__global__ void __launch_bounds__(kEvalT) eval_span_kernel(const Fr* __restrict__ a, uint64_t n,
-    const Fr* __restrict__ d_u, Fr* __restrict__ partial)
+// pw[t] = x^t for t <= kEvalT (whole work-group): with pw[0..h] known, threads
+// i < h set pw[h + 1 + i] = pw[1 + i] * pw[h].
+__device__ __forceinline__ void lds_powers(const Fr& x, Fr* pw)
 {
-    __shared__ Fr sh[kEvalT];
-    __shared__ Fr sh_big_u, sh_base_pow;
-    const uint64_t base = (uint64_t)blockIdx.x * kEvalSpan;
-    const Fr u = Fr::load(d_u);
     if (threadIdx.x == 0) {
-        Fr big = u;
-        for (unsigned s = 1; s < kEvalT; s <<= 1)
-            big = Fr::sqr(big);
-        sh_big_u = big;
-        sh_base_pow = Fr::pow_u64(u, base);
+        pw[0] = Fr::one();
+        pw[1] = x;
     }
     __syncthreads();
-    const Fr big_u = sh_big_u;
-    const uint64_t rem = n - base;
-    const unsigned rows = rem >= kEvalSpan ? kEvalM : (unsigned)((rem + kEvalT - 1) / kEvalT);
-    Fr acc = Fr::zero();
-    for (int m = (int)rows - 1; m >= 0; --m) {
-        const uint64_t k = base + (uint64_t)m * kEvalT + threadIdx.x;
-        acc = acc * big_u;
-        if (k < n)
-            acc = acc + Fr::load(a + k);
+    for (unsigned h = 1; h < kEvalT; h <<= 1) {
+        if (threadIdx.x < h)
+            pw[h + 1 + threadIdx.x] = pw[1 + threadIdx.x] * pw[h];
+        __syncthreads();
     }
-    acc = acc * (sh_base_pow * Fr::pow_u64(u, threadIdx.x));
-    const Fr s = block_sum(acc, sh);
-    if (threadIdx.x == 0)
-        s.store(partial + blockIdx.x);
 }
 
-__global__ void __launch_bounds__(kT) sum_kernel(const Fr* __restrict__ partial, uint64_t m, Fr* __restrict__ out)
+__device__ __forceinline__ Fr job_point(const EvalJob& jb) { return jb.du ? Fr::load(jb.du) : jb.u; }
+
+// grid (max blocks, jobs): partial[y * pstride + x] = P_x of job y
+__global__ void __launch_bounds__(kEvalT) eval_span_kernel(const EvalJobs jobs, uint64_t pstride,
+    Fr* __restrict__ partial)
 {
-    __shared__ Fr sh[kT];
+    __shared__ Fr pw[kEvalT + 1];
+    __shared__ Fr sh[kEvalT];
+    const EvalJob& jb = jobs.j[blockIdx.y];
+    const uint64_t base = (uint64_t)blockIdx.x * kEvalSpan;
+    if (base >= jb.n)
+        return; // uniform over the group
+    lds_powers(job_point(jb), pw);
+    const Fr big_u = pw[kEvalT];
+    const uint64_t rem = jb.n - base;
+    const unsigned rows = rem >= kEvalSpan ? kEvalRows : (unsigned)((rem + kEvalT - 1) / kEvalT);
+    const Fr* a = jb.a + base;
     Fr acc = Fr::zero();
-    for (uint64_t i = threadIdx.x; i < m; i += kT)
-        acc = acc + Fr::load(partial + i);
+    for (int m = (int)rows - 1; m >= 0; --m) {
+        const uint64_t k = (uint64_t)m * kEvalT + threadIdx.x;
+        acc = acc * big_u;
+        if (k < rem)
+            acc = acc + Fr::load(a + k);
+    }
+    acc = acc * pw[threadIdx.x];
     const Fr s = block_sum(acc, sh);
     if (threadIdx.x == 0)
-        s.store(out);
+        s.store(partial + blockIdx.y * pstride + blockIdx.x);
+}
+
+// one work-group per job: out = sum_b P_b V^b with V = u^kEvalSpan; thread t
+// takes blocks [t q, t q + q) by Horner in V and scales by (V^q)^t
+__global__ void __launch_bounds__(kEvalT) eval_combine_kernel(const EvalJobs jobs, uint64_t pstride,
+    const Fr* __restrict__ partial)
+{
+    __shared__ Fr pw[kEvalT + 1];
+    __shared__ Fr sh[kEvalT];
+    const EvalJob& jb = jobs.j[blockIdx.x];
+    const uint64_t nb = eval_blocks(jb.n);
+    lds_powers(job_point(jb), pw);
+    Fr v = pw[kEvalT];
+#pragma unroll
+    for (unsigned i = 0; i < kEvalLogRows; ++i)
+        v = Fr::sqr(v);
+    const uint64_t q = (nb + kEvalT - 1) / kEvalT;
+    const Fr w = Fr::pow_u64(v, q ? q : 1);
+    __syncthreads(); // every thread has read pw[kEvalT]
+    lds_powers(w, pw);
+    const Fr* p = partial + blockIdx.x * pstride;
+    Fr acc = Fr::zero();
+    const uint64_t b0 = threadIdx.x * q;
+    for (uint64_t r = q; r-- > 0;) {
+        acc = acc * v;
+        if (b0 + r < nb)
+            acc = acc + Fr::load(p + b0 + r);
+    }
+    acc = acc * pw[threadIdx.x];
+    const Fr s = block_sum(acc, sh);
+    if (threadIdx.x == 0)
+        s.store(jb.out);
+}
+
+// jobs[0, count) evaluated into their out slots (count <= kEvalMaxJobs); partial
+// holds count * pstride elements, pstride >= eval_blocks of every job
+RustError eval_jobs(const EvalJobs& jobs, uint32_t count, uint64_t pstride, Fr* partial, hipStream_t stream)
+{
+    if (count == 0)
+        return ok();
+    uint64_t maxb = 0;
+    for (uint32_t i = 0; i < count; ++i)
+        if (eval_blocks(jobs.j[i].n) > maxb)
+            maxb = eval_blocks(jobs.j[i].n);
+    if (maxb) {
+        eval_span_kernel<<<dim3((unsigned)maxb, count), kEvalT, 0, stream>>>(jobs, pstride, partial);
+        H2_LAUNCH_CHECK();
+    }
+    eval_combine_kernel<<<count, kEvalT, 0, stream>>>(jobs, pstride, partial);
+    H2_LAUNCH_CHECK();
+    return ok();
 }
 
 // p(u) into *out (device); u already on the device
 RustError eval_device(const Fr* a, uint64_t n, const Fr* d_u, const EvalBufs& b, Fr* out, hipStream_t stream)
 {
-    const uint64_t blocks = eval_blocks(n);
-    if (blocks) {
-        eval_span_kernel<<<(unsigned)blocks, kEvalT, 0, stream>>>(a, n, d_u, b.partial);
-        H2_LAUNCH_CHECK();
-    }
-    sum_kernel<<<1, kT, 0, stream>>>(b.partial, blocks, out);
-    H2_LAUNCH_CHECK();
-    return ok();
+    EvalJobs jobs;
+    jobs.j[0] = EvalJob { a, n, d_u, Fr::zero(), out };
+    return eval_jobs(jobs, 1, eval_blocks(n), b.partial, stream);
 }
 
 // ---------------------------------------------------------- elementwise
@@ -500,6 +571,47 @@ RustError _halo2_eval_polynomial(const void* d_poly, const FFITraitObject* point
     if (e.code)
         return e;
     *d_result_out = b.result;
+    return ok();
+}
+
+uint64_t h2amd_eval_polynomial_batch_device_workspace_size(const uint64_t* lengths, uint32_t count)
+{
+    uint64_t maxb = 1;
+    for (uint32_t i = 0; i < count; ++i)
+        if (eval_blocks(lengths[i]) > maxb)
+            maxb = eval_blocks(lengths[i]);
+    const uint64_t per = count < kEvalMaxJobs ? (count ? count : 1) : kEvalMaxJobs;
+    return align_up(32 * maxb * per, 256) + 256;
+}
+
+RustError h2amd_eval_polynomial_batch_device(const void* const* d_polys, const uint64_t* lengths, const void* h_points,
+    uint32_t count, void* d_results, void* scratch, uint64_t scratch_bytes, void* stream_)
+{
+    hipStream_t stream = (hipStream_t)stream_;
+    if (count == 0)
+        return ok();
+    uint64_t maxb = 1;
+    for (uint32_t i = 0; i < count; ++i)
+        if (eval_blocks(lengths[i]) > maxb)
+            maxb = eval_blocks(lengths[i]);
+    const uint64_t per = count < kEvalMaxJobs ? count : kEvalMaxJobs;
+    Workspace ws(scratch, scratch_bytes);
+    Fr* partial = (Fr*)ws.take(32 * maxb * per);
+    if (ws.overflow)
+        return fail((int)hipErrorInvalidValue, "h2amd_eval_polynomial_batch_device: workspace of %lu bytes is too small",
+            (unsigned long)scratch_bytes);
+    for (uint32_t c0 = 0; c0 < count; c0 += kEvalMaxJobs) {
+        const uint32_t m = count - c0 < kEvalMaxJobs ? count - c0 : kEvalMaxJobs;
+        EvalJobs jobs;
+        for (uint32_t i = 0; i < m; ++i) {
+            Fr u;
+            memcpy(u.v, (const uint8_t*)h_points + 32ull * (c0 + i), 32);
+            jobs.j[i] = EvalJob { (const Fr*)d_polys[c0 + i], lengths[c0 + i], nullptr, u, (Fr*)d_results + c0 + i };
+        }
+        RustError e = eval_jobs(jobs, m, maxb, partial, stream);
+        if (e.code)
+            return e;
+    }
     return ok();
 }
 

@@ -94,6 +94,8 @@ SIGNATURES = {
     "_halo2_permutation_product_device_inputs_workspace_size": (U64, [U64]),
     "_halo2_permutation_product_device_inputs": (RustError,
                                                  [OBJP, OBJP, OBJP, P, P, P, P, P, SZ, SZ, SZ, P, U64, P]),
+    "h2amd_eval_polynomial_batch_device_workspace_size": (U64, [P, U32]),
+    "h2amd_eval_polynomial_batch_device": (RustError, [P, P, P, U32, P, P, U64, P]),
     "_halo2_grand_product_max_len": (SZ, [SZ, U64]),
     "_halo2_grand_product_workspace_size": (U64, [U64]),
     "_halo2_grand_product": (RustError, [OBJP, OBJP, OBJP, SZ, SZ, P, U64, P]),

@@ -69,23 +69,20 @@ def eval_polynomial_gpu(poly, point: int) -> np.ndarray:
 
 def batch_eval_polynomial_device_out(d_polys: list[torch.Tensor], points: list[int]) -> torch.Tensor:
     """polynomial_ops.rs:142-222: device polys in, a device buffer of results out
-    (slot i = d_polys[i](points[i])), one scratch reused across the batch."""
+    (slot i = d_polys[i](points[i])), all polynomials in one batched launch sequence
+    (`h2amd_eval_polynomial_batch_device`)."""
     assert len(d_polys) == len(points) and d_polys, "empty batch"
     lib = _lib.load()
-    out = dv.empty_u64(4 * len(d_polys))
-    max_len = max(_len(p) for p in d_polys)
-    scratch = dv.empty_bytes(int(lib._halo2_eval_polynomial_workspace_size(max_len)))
-    base = dv.ptr(scratch)
-    words = scratch.view(torch.int64)
-    for i, (p, x) in enumerate(zip(d_polys, points)):
-        n = _len(p)
-        ws = int(lib._halo2_eval_polynomial_workspace_size(n))
-        pt = _scalar_host(x)
-        res = ctypes.c_void_p()
-        _lib.check(lib._halo2_eval_polynomial(dv.ptr(p), ctypes.byref(_hobj(pt)), ctypes.byref(res), n, base, ws,
-                                              dv.stream()), "_halo2_eval_polynomial")
-        off = (res.value - base) // 8
-        out[4 * i:4 * i + 4].copy_(words[off:off + 4])  # stream-ordered before the next evaluation
+    count = len(d_polys)
+    out = dv.empty_u64(4 * count)
+    lens = np.ascontiguousarray([_len(p) for p in d_polys], dtype=np.uint64)
+    pts = np.ascontiguousarray(np.stack([mont_limbs(x % R_MOD) for x in points]))
+    ptrs = (ctypes.c_void_p * count)(*[dv.ptr(p) for p in d_polys])
+    ws = int(lib.h2amd_eval_polynomial_batch_device_workspace_size(lens.ctypes.data, count))
+    scratch = dv.empty_bytes(ws)
+    _lib.check(lib.h2amd_eval_polynomial_batch_device(ptrs, lens.ctypes.data, pts.ctypes.data, count, dv.ptr(out),
+                                                      dv.ptr(scratch), ws, dv.stream()),
+               "h2amd_eval_polynomial_batch_device")
     return out
 
 

@@ -311,6 +311,14 @@ uint64_t _halo2_eval_polynomial_batch_workspace_size(uint64_t poly_length, uint6
 RustError _halo2_eval_polynomial_batch(const FFITraitObject* poly_in_many, const FFITraitObject* eval_points,
     const FFITraitObject* eval_result, size_t poly_offset, size_t poly_length, size_t batch_size, void* scratch,
     uint64_t scratch_bytes, void* stream);
+/* No counterpart in the reference (it evaluates one device polynomial per
+ * _halo2_eval_polynomial call, src/cuda/polynomial_ops.rs:142-222): count device
+ * polynomials (d_polys[i] with lengths[i] coefficients) at host points (32-byte
+ * Montgomery elements) into d_results[i] (device, 32 bytes each), in one pair of launches
+ * per 32 polynomials; stream-ordered, no synchronisation. */
+uint64_t h2amd_eval_polynomial_batch_device_workspace_size(const uint64_t* lengths, uint32_t count);
+RustError h2amd_eval_polynomial_batch_device(const void* const* d_polys, const uint64_t* lengths,
+    const void* h_points, uint32_t count, void* d_results, void* scratch, uint64_t scratch_bytes, void* stream);
 /* culib.rs:401-405 ; cuda/src/multiopen.cu:47-57 */
 size_t _halo2_multiopen_poly_max_len(size_t poly_length, size_t batch_size, uint64_t free_bytes);
 /* culib.rs:407-410 ; multiopen.cu:65-71 */

@@ -28,7 +28,8 @@ def _rand_int(seed):
     return random.Random(seed).randrange(R_MOD)
 
 
-EVAL_LENGTHS = [1, 2, 63, 255, 256, 16383, 16384, 16385, 1 << 16, (1 << 20) + 3]
+EVAL_LENGTHS = [1, 2, 63, 255, 256, 4095, 4096, 4097, 16383, 16384, 16385, 1 << 16, (1 << 20) + 3,
+                (1 << 22) + 4096 * 256 + 1]
 
 
 @pytest.mark.parametrize("n", EVAL_LENGTHS)
@@ -53,6 +54,25 @@ def test_eval_polynomial_batch_device_out(lib, co):
     got = batch_eval_polynomial_d2h([dv.to_device(p) for p in polys], pts)
     exp = np.stack([co.eval_poly(p, _fr(co, x)) for p, x in zip(polys, pts)])
     assert (got == exp).all()
+
+
+def test_eval_polynomial_batch_device_uneven(lib, co):
+    """One batched launch sequence over 41 polynomials (more than one launch's 32) of
+    uneven lengths, empty ones included (p(u) = 0), points 0, 1 and random."""
+    import torch
+
+    from halo2_amd import device as dv
+    from halo2_amd.polyops import batch_eval_polynomial_d2h
+
+    base = [0, 1, 2, 255, 256, 257, 4095, 4096, 4097, 8193, 1 << 20, 300000, 0, 7]
+    lens = (base * 3)[:41]
+    polys = [rand_fr(co, n, 900 + i) if n else np.zeros((0, 4), np.uint64) for i, n in enumerate(lens)]
+    pts = [[0, 1][i] if i < 2 else _rand_int(950 + i) for i in range(len(lens))]
+    d = [dv.to_device(p) if len(p) else torch.zeros(0, dtype=torch.int64, device=dv.device()) for p in polys]
+    got = batch_eval_polynomial_d2h(d, pts)
+    for i, (p, x) in enumerate(zip(polys, pts)):
+        exp = co.eval_poly(p, _fr(co, x)) if len(p) else np.zeros(4, np.uint64)
+        assert (got[i] == exp).all(), f"poly {i} n={lens[i]}"
 
 
 @pytest.mark.parametrize("free_bytes", [None, 3 * 32 * 1000])
