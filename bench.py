@@ -216,7 +216,8 @@ def batch_measurements(msm, torch, seed: int = 2024, count: int = 32) -> dict:
 def proof_measurements(msm, torch, k: int, reps: int) -> dict:
     """End-to-end proofs of the standard-gate circuit at 2^k rows (halo2_amd.prover:
     5 advice columns, copy cycles, a 16-bit range lookup, SHPLONK opening), one after
-    another on one GPU with the proving key and SRS resident: proofs/s, p50/p99 latency
+    another on one GPU with the proving key, the SRS and its fixed-base tables resident
+    (built by the untimed warm-up proof): proofs/s, p50/p99 latency
     and the per-phase split of the median proof. Every proof's self-checks must hold."""
     import numpy as np
 
@@ -224,9 +225,8 @@ def proof_measurements(msm, torch, k: int, reps: int) -> dict:
 
     circuit = prover.synthesize_standard(k, 5)
     pk = prover.keygen(circuit)
-    params = msm.ParamsKZG(k, 0x5EED, fixed_base=False)
-    _ = params.g_lagrange_device
-    prover.create_proof(params, pk, circuit, seed=99)  # warm-up
+    params = msm.ParamsKZG(k, 0x5EED)
+    prover.create_proof(params, pk, circuit, seed=99)  # warm-up: builds g_lagrange and the fixed-base tables
     lat, runs = [], []
     ta = time.perf_counter()
     for i in range(reps):

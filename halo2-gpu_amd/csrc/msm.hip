@@ -76,9 +76,9 @@ struct MsmPlan {
     uint64_t nbase; // bases converted to the 2^261 scale (a batch shares one base array)
     bool glv; // scalars split k = k1 + lambda k2: 2 nbase virtual points of 128-bit scalars
     uint64_t pts_count; // bases held in the 2^261 scale: nbase, or 2 nbase with the endomorphism images
-    uint32_t J; // MSMs in the pipeline (1 unless batched); each has its own W bucket sets
+    uint32_t J; // MSMs in the pipeline (1 unless batched); each has its own W bucket sets (1 with a table)
     uint32_t c, W, B;
-    uint32_t Wb; // bucket windows: W, or 1 when the windows share buckets (precomputed bases)
+    uint32_t Wb; // bucket windows: J W, or J when the windows share buckets (precomputed bases)
     uint64_t table_n; // precomputed bases: row stride of the table (0 = plain bases)
     uint64_t nb; // Wb * B buckets
     uint32_t FB, CB; // round-2 (fine) and round-1 (coarse) bucket bits, FB + CB = c - 1
@@ -152,7 +152,7 @@ uint32_t plain_msm_width(uint64_t npoints)
 }
 
 // n points (virtual range), window width c, J MSMs; table_n > 0 selects
-// precomputed bases (one MSM, shared buckets).
+// precomputed bases (each MSM's windows share its bucket set).
 MsmPlan plan_core(uint64_t n, uint32_t c, uint32_t J, uint64_t table_n, uint32_t bits = 255)
 {
     MsmPlan p {};
@@ -163,7 +163,7 @@ MsmPlan plan_core(uint64_t n, uint32_t c, uint32_t J, uint64_t table_n, uint32_t
     p.table_n = table_n;
     p.c = c;
     p.W = (bits + p.c - 1) / p.c;
-    p.Wb = table_n ? 1u : J * p.W;
+    p.Wb = table_n ? J : J * p.W;
     p.B = 1u << (p.c - 1);
     p.nb = (uint64_t)p.Wb * p.B;
     p.FB = p.c - 1 < MAX_FB ? p.c - 1 : MAX_FB;
@@ -216,9 +216,10 @@ MsmPlan make_plan(uint64_t n, uint64_t table_n = 0)
 
 uint64_t align_tile(uint64_t n) { return (n + R1_TILE - 1) / R1_TILE * R1_TILE; }
 
-// J MSMs over one shared base array: one width for all (from the mean job
-// size), each job's points at a tile-aligned virtual offset.
-MsmPlan make_batch_plan(const uint64_t* lengths, uint32_t J)
+// J MSMs over one shared base array (or one precomputed table of table_n
+// points): one width for all (from the mean job size, or the table's), each
+// job's points at a tile-aligned virtual offset.
+MsmPlan make_batch_plan(const uint64_t* lengths, uint32_t J, uint64_t table_n = 0)
 {
     uint64_t nv = 0, nmax = 0;
     for (uint32_t j = 0; j < J; ++j) {
@@ -227,10 +228,10 @@ MsmPlan make_batch_plan(const uint64_t* lengths, uint32_t J)
             nmax = lengths[j];
     }
     const uint64_t mean = nv / J ? nv / J : 1;
-    uint32_t c = plan_width(mean, false);
-    if (c > 16) // J * W * 2^(c-1) buckets of 128 B: 4 GiB at most for 64 jobs
+    uint32_t c = table_n ? plan_width(table_n, true) : plan_width(mean, false);
+    if (!table_n && c > 16) // J * W * 2^(c-1) buckets of 128 B: 4 GiB at most for 64 jobs
         c = 16;
-    MsmPlan p = plan_core(nv ? nv : 1, c, J, 0);
+    MsmPlan p = plan_core(nv ? nv : 1, c, J, table_n);
     p.n = nv;
     p.ntiles = (uint32_t)(nv / R1_TILE);
     p.nbase = nmax;
@@ -622,7 +623,7 @@ __global__ void __launch_bounds__(R1_THREADS) r1_scatter_kernel(const uint32_t* 
             if (d != 0) {
                 const uint32_t bi = (uint32_t)(d < 0 ? -d : d) - 1u;
                 bin[k] = bi >> sp.FB;
-                const uint32_t idx = sp.table_n ? (uint32_t)(w * sp.table_n + i) : (uint32_t)(i - tr.v0);
+                const uint32_t idx = (uint32_t)((sp.table_n ? w * sp.table_n : 0) + (i - tr.v0));
                 // the point's own sign (a negative GLV half) rides in bit 31 of scalar word 7
                 const bool neg = (d < 0) != ((s[k][7] >> 31) != 0);
                 ent[k] = ((uint64_t)(bi & fmask) << 32) | idx | (neg ? 0x80000000u : 0u);
@@ -1544,17 +1545,17 @@ uint64_t h2amd_multiexp_batch_workspace_size(const uint64_t* lengths, uint32_t c
     return sa.s.bytes + 4096;
 }
 
-RustError h2amd_multiexp_batch(const void* const* d_scalars, const uint64_t* lengths, uint32_t count,
-    const void* d_bases, void* h_out, void* scratch, uint64_t scratch_bytes, void* stream_)
+// J MSMs in one pipeline over shared bases (table_n = 0) or one precomputed table
+static RustError batch_entry(const char* who, const void* const* d_scalars, const uint64_t* lengths, uint32_t count,
+    const void* d_bases, uint64_t table_n, void* h_out, void* scratch, uint64_t scratch_bytes, hipStream_t stream)
 {
-    hipStream_t stream = (hipStream_t)stream_;
-    H2_CHECK_ARG(count <= MAX_JOBS, "h2amd_multiexp_batch: %u MSMs exceed %u per batch", count, MAX_JOBS);
+    H2_CHECK_ARG(count <= MAX_JOBS, "%s: %u MSMs exceed %u per batch", who, count, MAX_JOBS);
     if (count == 0)
         return ok();
     uint64_t* o = (uint64_t*)h_out;
     uint64_t nv = 0;
     for (uint32_t j = 0; j < count; ++j) {
-        H2_CHECK_ARG(lengths[j] < (1ull << 31), "h2amd_multiexp_batch: length %lu of MSM %u exceeds 2^31",
+        H2_CHECK_ARG(lengths[j] < (1ull << 31), "%s: length %lu of MSM %u exceeds 2^31", who,
             (unsigned long)lengths[j], j);
         nv += align_tile(lengths[j]);
     }
@@ -1565,14 +1566,15 @@ RustError h2amd_multiexp_batch(const void* const* d_scalars, const uint64_t* len
             memcpy(o + 12 * j + 4, one.v, 32);
         return ok();
     }
-    const MsmPlan p = make_batch_plan(lengths, count);
-    H2_CHECK_ARG(p.n * p.W < (1ull << 32), "h2amd_multiexp_batch: %lu padded points too many for 32-bit offsets",
+    const MsmPlan p = make_batch_plan(lengths, count, table_n);
+    H2_CHECK_ARG(p.n * p.W < (1ull << 32), "%s: %lu padded points too many for 32-bit offsets", who,
         (unsigned long)p.n);
+    H2_CHECK_ARG(table_n * p.W < (1ull << 31), "%s: table too large", who);
     Workspace ws(scratch, scratch_bytes);
     MsmBuffers b {};
     carve(ws, p, &b);
     if (ws.overflow)
-        return fail((int)hipErrorInvalidValue, "h2amd_multiexp_batch: workspace of %lu bytes is too small",
+        return fail((int)hipErrorInvalidValue, "%s: workspace of %lu bytes is too small", who,
             (unsigned long)scratch_bytes);
     std::vector<BatchJob> jobs(count);
     std::vector<uint32_t> tile_job(p.ntiles);
@@ -1600,6 +1602,35 @@ RustError h2amd_multiexp_batch(const void* const* d_scalars, const uint64_t* len
     // also keeps the host job tables alive until their copies are done
     H2_TRY(hipStreamSynchronize(stream));
     return ok();
+}
+
+RustError h2amd_multiexp_batch(const void* const* d_scalars, const uint64_t* lengths, uint32_t count,
+    const void* d_bases, void* h_out, void* scratch, uint64_t scratch_bytes, void* stream_)
+{
+    return batch_entry("h2amd_multiexp_batch", d_scalars, lengths, count, d_bases, 0, h_out, scratch, scratch_bytes,
+        (hipStream_t)stream_);
+}
+
+uint64_t h2amd_multiexp_batch_precomputed_workspace_size(const uint64_t* lengths, uint32_t count, uint64_t table_n)
+{
+    if (count == 0 || count > MAX_JOBS || table_n == 0)
+        return 0;
+    const MsmPlan p = make_batch_plan(lengths, count, table_n);
+    SizerAdapter sa;
+    carve(sa, p, nullptr);
+    return sa.s.bytes + 4096;
+}
+
+RustError h2amd_multiexp_batch_precomputed(const void* const* d_scalars, const uint64_t* lengths, uint32_t count,
+    const void* d_table, uint64_t table_n, void* h_out, void* scratch, uint64_t scratch_bytes, void* stream_)
+{
+    H2_CHECK_ARG(table_n > 0, "h2amd_multiexp_batch_precomputed: empty table");
+    for (uint32_t j = 0; j < count; ++j)
+        H2_CHECK_ARG(lengths[j] <= table_n,
+            "h2amd_multiexp_batch_precomputed: length %lu of MSM %u exceeds the table's %lu points",
+            (unsigned long)lengths[j], j, (unsigned long)table_n);
+    return batch_entry("h2amd_multiexp_batch_precomputed", d_scalars, lengths, count, d_table, table_n, h_out,
+        scratch, scratch_bytes, (hipStream_t)stream_);
 }
 
 // ----------------------------------- Lagrange bases (no reference counterpart)

@@ -155,6 +155,8 @@ SIGNATURES = {
     "h2amd_g1_compress": (RustError, [P, P, U64, P]),
     "h2amd_g1_decompress": (RustError, [P, P, U64, P, P, P, U64, P]),
     "h2amd_multiexp_batch": (RustError, [P, P, U32, P, P, P, U64, P]),
+    "h2amd_multiexp_batch_precomputed_workspace_size": (U64, [P, U32, U64]),
+    "h2amd_multiexp_batch_precomputed": (RustError, [P, P, U32, P, U64, P, P, U64, P]),
     "h2amd_random_fr": (RustError, [P, U64, U64, U64, P]),
     "h2amd_g1_mul_generator_workspace_size": (U64, [U64]),
     "h2amd_g1_mul_generator": (RustError, [P, P, U64, P, U64, P]),

@@ -138,6 +138,25 @@ def multiexp_batch_device(d_scalars: list[torch.Tensor], d_bases: torch.Tensor,
     return out[:count]
 
 
+def multiexp_batch_precomputed(d_scalars: list[torch.Tensor], table: torch.Tensor, table_n: int,
+                               lengths: list[int] | None = None) -> np.ndarray:
+    """`h2amd_multiexp_batch_precomputed`: the batch of multiexp_batch_device over a
+    fixed-base table of table_n bases (msm_precompute); (count, 12) uint64."""
+    lib = _lib.load()
+    count = len(d_scalars)
+    if lengths is None:
+        lengths = [x.numel() // 4 for x in d_scalars]
+    lens = np.ascontiguousarray(lengths, dtype=np.uint64)
+    ptrs = (ctypes.c_void_p * max(1, count))(*[dv.ptr(x) for x in d_scalars])
+    ws = int(lib.h2amd_multiexp_batch_precomputed_workspace_size(lens.ctypes.data, count, table_n))
+    scratch = dv.empty_bytes(ws)
+    out = np.zeros((max(1, count), 12), dtype=np.uint64)
+    _lib.check(lib.h2amd_multiexp_batch_precomputed(ptrs, lens.ctypes.data, count, dv.ptr(table), table_n,
+                                                    out.ctypes.data, dv.ptr(scratch), ws, dv.stream()),
+               "h2amd_multiexp_batch_precomputed")
+    return out[:count]
+
+
 def g_to_lagrange_device(d_g: torch.Tensor, k: int, omega: int | None = None) -> torch.Tensor:
     """`h2amd_g1_to_lagrange` (halo2-axiom g_to_lagrange, used by ParamsKZG::from_parts and
     downsize, commitment.rs:205-228, :436-450): the Lagrange bases n^-1 sum_j w^-ij g_j of
@@ -307,12 +326,27 @@ class ParamsKZG:
         Lagrange basis."""
         return self._commit_resident(d_values, "g_lagrange")
 
+    def _commit_many(self, d_polys: list[torch.Tensor], which: str) -> np.ndarray:
+        bases = self.g_device if which == "g" else self.g_lagrange_device
+        if self.fixed_base and which not in self._tables:
+            self._tables[which] = msm_precompute(bases, self.n)
+        outs = []
+        for i in range(0, len(d_polys), 64):
+            part = d_polys[i:i + 64]
+            outs.append(multiexp_batch_precomputed(part, self._tables[which], self.n) if self.fixed_base
+                        else multiexp_batch_device(part, bases))
+        return np.concatenate(outs) if outs else np.zeros((0, 12), dtype=np.uint64)
+
     def commit_many_device(self, d_polys: list[torch.Tensor]) -> np.ndarray:
         """KZG commitments of many device polynomials (any lengths <= n) against the
-        resident SRS in one batched MSM pipeline, at most 64 per launch sequence;
-        the same points commit_device gives one by one."""
-        outs = [multiexp_batch_device(d_polys[i:i + 64], self.g_device) for i in range(0, len(d_polys), 64)]
-        return np.concatenate(outs) if outs else np.zeros((0, 12), dtype=np.uint64)
+        resident SRS in one batched MSM pipeline (over the fixed-base table unless
+        fixed_base is off), at most 64 per launch sequence; the same points
+        commit_device gives one by one."""
+        return self._commit_many(d_polys, "g")
+
+    def commit_lagrange_many_device(self, d_values: list[torch.Tensor]) -> np.ndarray:
+        """commit_lagrange of many device Lagrange vectors in one batched pipeline."""
+        return self._commit_many(d_values, "g_lagrange")
 
     def commit_lagrange(self, h_values: np.ndarray) -> np.ndarray:
         """commitment.rs:456-466: host Lagrange values, resident Lagrange basis."""

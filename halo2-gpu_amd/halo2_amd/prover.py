@@ -244,7 +244,7 @@ def create_proof(params: msm.ParamsKZG, pk: ProvingKey, circuit: Circuit, seed: 
         t0 = t
 
     def commit_lagrange_many(cols):
-        return msm.multiexp_batch_device(cols, params.g_lagrange_device)
+        return params.commit_lagrange_many_device(cols)
 
     # 1. advice (prover.rs:282-466): commitments in Lagrange form, then coefficients
     for pt in commit_lagrange_many(circuit.advice):
@@ -303,7 +303,7 @@ def create_proof(params: msm.ParamsKZG, pk: ProvingKey, circuit: Circuit, seed: 
     top = num.view(-1, 4)[(CS_DEGREE - 1) * n:]
     checks["h_is_a_polynomial"] = not bool(top.any().item())
     pieces = [num[4 * i * n: 4 * (i + 1) * n] for i in range(CS_DEGREE - 1)]
-    for pt in msm.multiexp_batch_device(pieces, params.g_device):
+    for pt in params.commit_many_device(pieces):
         tr.write_point(pt)
     phase("quotient")
 

@@ -429,6 +429,13 @@ RustError h2amd_fft_batch_to_device(uint32_t ntt_type, uint32_t count, const uin
 uint64_t h2amd_multiexp_batch_workspace_size(const uint64_t* lengths, uint32_t count);
 RustError h2amd_multiexp_batch(const void* const* d_scalars, const uint64_t* lengths, uint32_t count,
     const void* d_bases, void* h_out, void* scratch, uint64_t scratch_bytes, void* stream);
+/* The same batch over a precomputed table (h2amd_msm_precompute of table_n bases, rows
+ * 2^(c w) P): every lengths[j] <= table_n; each MSM's windows share one bucket set, so
+ * there is no doubling chain per MSM. Results equal the single-MSM ones bit for bit.
+ * Synchronises. */
+uint64_t h2amd_multiexp_batch_precomputed_workspace_size(const uint64_t* lengths, uint32_t count, uint64_t table_n);
+RustError h2amd_multiexp_batch_precomputed(const void* const* d_scalars, const uint64_t* lengths, uint32_t count,
+    const void* d_table, uint64_t table_n, void* h_out, void* scratch, uint64_t scratch_bytes, void* stream);
 
 /* ======================= SRS point formats (G1 halves of ParamsKZG::write_custom /
  * read_custom, src/poly/kzg/commitment.rs:317-414; no FFI counterpart: the reference

@@ -331,6 +331,34 @@ def test_msm_batch_uneven(lib, co):
             assert (got[j] == msm.multiexp_device_scalars_device_bases(d[j], pts, n)).all()
 
 
+def test_msm_batch_precomputed_uneven(lib, co):
+    """The batch over a fixed-base table: uneven lengths (empty, tile edges, dense,
+    half-zero, the table's full length) and lookup-style small scalars, each equal to
+    its closed form; a job longer than the table is refused."""
+    import torch
+
+    from halo2_amd import _lib, device as dv, fields as F, msm
+
+    lengths = [0, 1, 4095, 4096, 4097, 12345, 1 << 16, 1, 77, 0, 5000, (1 << 16) - 3]
+    nmax = 1 << 16
+    print(f"msm batch precomputed {len(lengths)} jobs, table {nmax}", flush=True)
+    pts, scal, exp = _batch_case(co, lengths, 83, nmax)
+    small = np.stack([F.mont_limbs(int(v)) for v in np.random.default_rng(84).integers(0, 1 << 16, 3000)])
+    scal.append(small)
+    exp.append(co.msm(small, dv.to_host(pts, cols=8)[:3000]))
+    lengths.append(3000)
+    table = msm.msm_precompute(pts, nmax)
+    d = [dv.to_device(c) if len(c) else torch.zeros(4, dtype=torch.int64, device=pts.device) for c in scal]
+    got = msm.multiexp_batch_precomputed(d, table, nmax, lengths)
+    for j, n in enumerate(lengths):
+        if n == 0:
+            assert (got[j] == _identity()).all(), f"job {j} empty"
+            continue
+        assert (got[j] == exp[j]).all(), f"job {j} n={n}"
+    with pytest.raises(_lib.Halo2GpuError):
+        msm.multiexp_batch_precomputed([msm.random_fr(nmax + 1, 85)], table, nmax)
+
+
 def test_msm_batch_many_jobs_and_limits(lib, co):
     """64 small jobs (the most one batch takes) against closed forms; 65 is refused and
     an all-empty batch gives identities."""

@@ -8,14 +8,14 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _prove(k, seed, mutate=None):
+def _prove(k, seed, mutate=None, fixed_base=False):
     from halo2_amd import fields as F, msm, prover
 
     circuit = prover.synthesize_standard(k, seed, table_bits=8)
     if mutate:
         mutate(circuit)
     pk = prover.keygen(circuit)
-    params = msm.ParamsKZG(k, 0x5EED_0F_7A0 % F.R_MOD, fixed_base=False)
+    params = msm.ParamsKZG(k, 0x5EED_0F_7A0 % F.R_MOD, fixed_base=fixed_base)
     return prover.create_proof(params, pk, circuit, seed=seed)
 
 
@@ -25,6 +25,8 @@ def test_proof_pipeline_self_checks(lib):
                              "h_is_a_polynomial": True}
     assert len(out["proof"]) > 0 and set(out["times_ms"]) >= {"advice", "quotient", "opening"}
     assert _prove(10, 3)["proof"] == out["proof"]  # deterministic under one seed
+    # the fixed-base tables give the same commitments, hence the same transcript
+    assert _prove(10, 3, fixed_base=True)["proof"] == out["proof"]
 
 
 def test_proof_pipeline_detects_bad_witness(lib):
