@@ -33,12 +33,16 @@ from .quotient import ColumnPool, GraphEvaluator, compress_expressions_with_runt
 @dataclass
 class LookupCommitted:
     """lookup::prover::Committed: the product polynomial (coefficients) and the permuted
-    input / table expressions (Lagrange values), plus the lookup's evaluator graph."""
+    input / table expressions (Lagrange values), plus the lookup's evaluator graph.
+    The permuted columns' coefficient forms, when the caller has them, save their
+    inverse transforms in every part."""
 
     graph: GraphEvaluator
     product_poly: torch.Tensor
     permuted_input: torch.Tensor
     permuted_table: torch.Tensor
+    permuted_input_poly: torch.Tensor | None = None
+    permuted_table_poly: torch.Tensor | None = None
 
 
 @dataclass
@@ -69,10 +73,15 @@ def lagrange_to_extend_part(domain, poly: torch.Tensor, extended_omega_factor: i
 def evaluate_h_device(domain, gates: GraphEvaluator, fixed_polys, l0, l_last, l_active, advice_polys,
                       instance_polys, challenges, y: int, beta: int, gamma: int, theta: int,
                       lookups: list[LookupCommitted] = (), permutation: PermutationCommitted | None = None,
-                      blinding_factors: int = 0, cs_degree: int = 3) -> torch.Tensor:
+                      blinding_factors: int = 0, cs_degree: int = 3, key_parts: dict | None = None) -> torch.Tensor:
     """Extended Lagrange values of h's numerator (before the vanishing division), one
     circuit; all polynomials are device tensors of n coefficients (permuted lookup
-    columns: n Lagrange values)."""
+    columns: n Lagrange values).
+
+    key_parts: a dict owned by the proving key that keeps, per part, the coset values of
+    the polynomials that do not change between proofs (fixed columns, l0, l_last,
+    l_active, sigma) -- the proving key's fixed_cosets / permutation cosets of halo2
+    (keygen.rs), 4 (nf + 3 + sigmas) x 32 n bytes of HBM; filled on first use."""
     lib = _lib.load()
     n = domain.n
     num_parts = domain.extended_len >> domain.k
@@ -80,13 +89,29 @@ def evaluate_h_device(domain, gates: GraphEvaluator, fixed_polys, l0, l_last, l_
     cur = 1
     nf, na = len(fixed_polys), len(advice_polys)
     value_parts = []
-    for _ in range(num_parts):
-        polys = list(fixed_polys) + [l0, l_last, l_active] + list(advice_polys) + list(instance_polys)
-        parts = domain.coeff_to_extended_part_many_device(polys, cur)
-        fixed = parts[:nf]
-        l0_e, ll_e, la_e = parts[nf:nf + 3]
-        advice = parts[nf + 3:nf + 3 + na]
-        instance = parts[nf + 3 + na:]
+    sigma_polys = list(permutation.sigma_polys) if permutation is not None and permutation.product_polys else []
+    for part in range(num_parts):
+        key = key_parts.get(part) if key_parts is not None else None
+        if key is None:
+            key = domain.coeff_to_extended_part_many_device(
+                list(fixed_polys) + [l0, l_last, l_active] + sigma_polys, cur)
+            if key_parts is not None:
+                key_parts[part] = key
+        fixed = key[:nf]
+        l0_e, ll_e, la_e = key[nf:nf + 3]
+        sig = key[nf + 3:]
+        # every per-proof polynomial of the part in one batched transform
+        per_proof = list(advice_polys) + list(instance_polys)
+        if permutation is not None and permutation.product_polys:
+            per_proof += list(permutation.product_polys)
+        for lk in lookups:
+            per_proof.append(lk.product_poly)
+            if lk.permuted_input_poly is not None and lk.permuted_table_poly is not None:
+                per_proof += [lk.permuted_input_poly, lk.permuted_table_poly]
+        parts = domain.coeff_to_extended_part_many_device(per_proof, cur)
+        advice = parts[:na]
+        instance = parts[na:na + len(instance_polys)]
+        rest = parts[na + len(instance_polys):]
         pool = ColumnPool(n)
         pool.try_init_device(fixed, [], advice, instance)
 
@@ -95,8 +120,7 @@ def evaluate_h_device(domain, gates: GraphEvaluator, fixed_polys, l0, l_last, l_
         if permutation is not None and permutation.product_polys:
             chunk_len = cs_degree - 2
             last_rotation = -(blinding_factors + 1)
-            zs = domain.coeff_to_extended_part_many_device(permutation.product_polys, cur)
-            sig = domain.coeff_to_extended_part_many_device(permutation.sigma_polys, cur)
+            zs, rest = rest[:len(permutation.product_polys)], rest[len(permutation.product_polys):]
             groups = {"fixed": fixed, "advice": advice, "instance": instance}
             cols = [groups[kind][idx] for kind, idx in permutation.columns]
             tz, ts, tc = _ptr_table(zs), _ptr_table(sig), _ptr_table(cols)
@@ -112,9 +136,12 @@ def evaluate_h_device(domain, gates: GraphEvaluator, fixed_polys, l0, l_last, l_
             for lk in lookups:
                 compress_expressions_with_runtime_constants_device(lk.graph, theta, beta, gamma, y, n, 1, pool,
                                                                    challenges, table)
-                pin = lagrange_to_extend_part(domain, lk.permuted_input, cur)
-                ptab = lagrange_to_extend_part(domain, lk.permuted_table, cur)
-                prod = domain.coeff_to_extended_part_many_device([lk.product_poly], cur)[0]
+                prod, rest = rest[0], rest[1:]
+                if lk.permuted_input_poly is not None and lk.permuted_table_poly is not None:
+                    (pin, ptab), rest = rest[:2], rest[2:]
+                else:
+                    pin = lagrange_to_extend_part(domain, lk.permuted_input, cur)
+                    ptab = lagrange_to_extend_part(domain, lk.permuted_table, cur)
                 _lib.check(lib._halo2_quotient_lookups(
                     dv.ptr(values), dv.ptr(table), dv.ptr(prod), dv.ptr(pin), dv.ptr(ptab), dv.ptr(l0_e),
                     dv.ptr(ll_e), dv.ptr(la_e), *[dv.ptr(x) for x in d_sc], n, stream), "_halo2_quotient_lookups")

@@ -174,6 +174,7 @@ class ProvingKey:
     sigma_polys: list  # coefficients
     gates: object = None
     lookup: object = None
+    key_parts: dict = field(default_factory=dict)  # coset values of the fixed part, per extended part
     timings: dict = field(default_factory=dict)
 
 
@@ -294,9 +295,9 @@ def create_proof(params: msm.ParamsKZG, pk: ProvingKey, circuit: Circuit, seed: 
     # 5. h(X) (evaluation.rs:1056-1417), divided by X^n - 1, in pieces of n
     num = evaluate_h_device(
         dom, pk.gates, pk.fixed_polys, pk.l0, pk.l_last, pk.l_active, advice_polys, circuit.instance, [], y, beta,
-        gamma, theta, lookups=[LookupCommitted(pk.lookup, z_lookup_poly, p_in, p_tab)],
+        gamma, theta, lookups=[LookupCommitted(pk.lookup, z_lookup_poly, p_in, p_tab, p_in_poly, p_tab_poly)],
         permutation=PermutationCommitted([z_perm_poly], [("advice", 0), ("advice", 1)], pk.sigma_polys),
-        blinding_factors=BLINDING_FACTORS, cs_degree=CS_DEGREE)
+        blinding_factors=BLINDING_FACTORS, cs_degree=CS_DEGREE, key_parts=pk.key_parts)
     dom.divide_by_vanishing_poly_device(num)
     ntt.fft_normal_device(ntt.IFFT, dom.extended_k, num, num, dom.extended_omega_inv, dom.extended_ifft_divisor)
     ntt.distribute_powers_zeta_device(num, [dom.g_coset_inv, dom.g_coset])

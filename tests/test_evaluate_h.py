@@ -108,6 +108,19 @@ def test_evaluate_h_vs_oracle(lib, co, k):
                                          [D(x) for x in perm["sigmas"]]),
         blinding_factors=blinding, cs_degree=cs_degree)
     assert (dv.to_host(got) == exp).all()
+    # the proving key's cached coset parts (filled, then reused) and the permuted
+    # columns given in coefficient form give the same values
+    key_parts = {}
+    pin_c, ptab_c = (dom.lagrange_to_coeff_device_input(D(x)) for x in (lk_pin, lk_ptab))
+    for _ in range(2):
+        again = evaluate_h_device(
+            dom, gates, [D(x) for x in fixed], D(l0), D(ll), D(la), [D(x) for x in advice],
+            [D(x) for x in instance], chals, y, beta, gamma, theta,
+            lookups=[LookupCommitted(lk_graph, D(lk_prod), D(lk_pin), D(lk_ptab), pin_c, ptab_c)],
+            permutation=PermutationCommitted([D(x) for x in perm["products"]], perm["columns"],
+                                             [D(x) for x in perm["sigmas"]]),
+            blinding_factors=blinding, cs_degree=cs_degree, key_parts=key_parts)
+        assert len(key_parts) == 4 and (dv.to_host(again) == exp).all()
 
 
 def test_evaluate_h_gate_degree(lib, co):
