@@ -105,9 +105,11 @@ uint32_t log2_floor(uint64_t n)
 // Window width: plain bases need W bucket sets (one per window) and a Horner
 // over windows, so c grows more slowly than log2 n; precomputed bases
 // (rows 2^(c w) P) share one bucket set, so the width can follow log2 n.
-// Window widths chosen by a tuner, per floor(log2 n) of a plain MSM (0 = use
-// the default rule). Set with h2amd_msm_set_widths before sizing and launching.
-uint8_t g_width_table[32] = { 0 };
+// Window widths chosen by a tuner (0 = use the default rule): entry i < 32 for
+// plain MSMs of floor(log2 n) = i, entry 32 + i for precomputed tables of
+// floor(log2 table_n) = i. Set with h2amd_msm_set_widths before sizing,
+// launching and building tables.
+uint8_t g_width_table[64] = { 0 };
 
 // n (virtual) points of `bits`-bit signed-digit scalars
 uint32_t default_width(uint64_t n, bool shared, uint32_t bits)
@@ -124,11 +126,9 @@ uint32_t default_width(uint64_t n, bool shared, uint32_t bits)
 
 uint32_t plan_width(uint64_t n, bool shared)
 {
-    if (!shared) {
-        const uint32_t tuned = g_width_table[log2_floor(n < 1 ? 1 : n) & 31];
-        if (tuned)
-            return tuned;
-    }
+    const uint32_t tuned = g_width_table[(log2_floor(n < 1 ? 1 : n) & 31) + (shared ? 32 : 0)];
+    if (tuned)
+        return tuned;
     return default_width(n, shared, 255);
 }
 
@@ -1705,15 +1705,17 @@ RustError h2amd_g1_to_lagrange(void* d_out, const void* d_g, uint32_t k, const v
 
 RustError h2amd_msm_set_widths(const uint8_t* widths, uint32_t count)
 {
-    H2_CHECK_ARG(count <= 32, "h2amd_msm_set_widths: %u entries exceed 32", count);
+    H2_CHECK_ARG(count <= 64, "h2amd_msm_set_widths: %u entries exceed 64", count);
     for (uint32_t i = 0; i < count; ++i)
-        H2_CHECK_ARG(widths[i] == 0 || (widths[i] >= 4 && widths[i] <= 20),
-            "h2amd_msm_set_widths: width %u for 2^%u outside [4, 20]", widths[i], i);
-    for (uint32_t i = 0; i < 32; ++i)
+        H2_CHECK_ARG(widths[i] == 0 || (widths[i] >= 4 && widths[i] <= (i < 32 ? 20 : 22)),
+            "h2amd_msm_set_widths: width %u for entry %u outside [4, %u]", widths[i], i, i < 32 ? 20u : 22u);
+    for (uint32_t i = 0; i < 64; ++i)
         g_width_table[i] = i < count ? widths[i] : 0;
     return ok();
 }
 
 uint32_t h2amd_msm_width(uint64_t n) { return plain_msm_width(n); }
+
+uint32_t h2amd_msm_table_width(uint64_t table_n) { return plan_width(table_n ? table_n : 1, true); }
 
 } // extern "C"

@@ -149,6 +149,7 @@ SIGNATURES = {
     "h2amd_g1_to_lagrange_workspace_size": (U64, [U32]),
     "h2amd_msm_set_widths": (RustError, [P, U32]),
     "h2amd_msm_width": (U32, [U64]),
+    "h2amd_msm_table_width": (U32, [U64]),
     "h2amd_ntt_set_tiles": (RustError, [P, U32]),
     "h2amd_g1_to_lagrange": (RustError, [P, P, U32, P, P, U64, P]),
     "h2amd_g1_validate": (RustError, [P, U64, P, P, P, U64, P]),

@@ -32,11 +32,15 @@ def _median_ms(fn, reps: int) -> float:
     return float(np.median(ts))
 
 
-def set_msm_widths(widths: dict[int, int]) -> None:
-    table = np.zeros(32, dtype=np.uint8)
+def set_msm_widths(widths: dict[int, int], table_widths: dict[int, int] | None = None) -> None:
+    """Plain MSM widths per log2 n and precomputed-table widths per log2 table_n (tables
+    already built stay valid only under the width they were built with)."""
+    table = np.zeros(64, dtype=np.uint8)
     for lg, c in widths.items():
         table[lg] = c
-    _lib.check(_lib.load().h2amd_msm_set_widths(table.ctypes.data, 32), "h2amd_msm_set_widths")
+    for lg, c in (table_widths or {}).items():
+        table[32 + lg] = c
+    _lib.check(_lib.load().h2amd_msm_set_widths(table.ctypes.data, 64), "h2amd_msm_set_widths")
 
 
 def set_ntt_tiles(tiles: dict[int, int]) -> None:

@@ -226,6 +226,7 @@ class ParamsKZG:
         # fixed-base tables (rows 2^(c w) of each basis), built on first commit
         self.fixed_base = fixed_base
         self._tables: dict[str, torch.Tensor] = {}
+        self._table_width: dict[str, int] = {}
         self._g2 = self._s_g2 = None
 
     @classmethod
@@ -247,6 +248,7 @@ class ParamsKZG:
         self._g_lagrange_device = g_lagrange_device
         self.fixed_base = fixed_base
         self._tables = {}
+        self._table_width = {}
         self._g2, self._s_g2 = g2, s_g2
         return self
 
@@ -261,6 +263,7 @@ class ParamsKZG:
         self.g_device = self.g_device[: 8 * self.n].clone()
         self._g_lagrange_device = g_to_lagrange_device(self.g_device, k)
         self._tables = {}
+        self._table_width = {}
 
     @property
     def g2(self):
@@ -286,9 +289,20 @@ class ParamsKZG:
         bases = self.g_device if which == "g" else self.g_lagrange_device
         if not self.fixed_base:
             return multiexp_device_scalars_device_bases(d_scalars, bases[: 8 * n], n)
+        return multiexp_precomputed(d_scalars, self._table(which), self.n, n)
+
+    def _table(self, which: str) -> torch.Tensor:
+        """The fixed-base table of g or g_lagrange, built on first use under the current
+        table width; a width changed since (h2amd_msm_set_widths) is refused."""
+        width = int(_lib.load().h2amd_msm_table_width(self.n))
         if which not in self._tables:
+            bases = self.g_device if which == "g" else self.g_lagrange_device
             self._tables[which] = msm_precompute(bases, self.n)
-        return multiexp_precomputed(d_scalars, self._tables[which], self.n, n)
+            self._table_width[which] = width
+        if self._table_width[which] != width:
+            raise RuntimeError(f"the {which} table was built with window width {self._table_width[which]}, "
+                               f"the library now uses {width}: rebuild the parameters")
+        return self._tables[which]
 
     @property
     def g_lagrange_device(self) -> torch.Tensor:
@@ -328,12 +342,10 @@ class ParamsKZG:
 
     def _commit_many(self, d_polys: list[torch.Tensor], which: str) -> np.ndarray:
         bases = self.g_device if which == "g" else self.g_lagrange_device
-        if self.fixed_base and which not in self._tables:
-            self._tables[which] = msm_precompute(bases, self.n)
         outs = []
         for i in range(0, len(d_polys), 64):
             part = d_polys[i:i + 64]
-            outs.append(multiexp_batch_precomputed(part, self._tables[which], self.n) if self.fixed_base
+            outs.append(multiexp_batch_precomputed(part, self._table(which), self.n) if self.fixed_base
                         else multiexp_batch_device(part, bases))
         return np.concatenate(outs) if outs else np.zeros((0, 12), dtype=np.uint64)
 

@@ -239,7 +239,7 @@ def create_proof(params: msm.ParamsKZG, pk: ProvingKey, circuit: Circuit, seed: 
 
     def phase(name):
         nonlocal t0
-        torch.cuda.synchronize()
+        torch.cuda.current_stream().synchronize()  # this proof's stream only: proofs may run side by side
         t = time.perf_counter()
         times[name] = round((t - t0) * 1e3, 3)
         t0 = t

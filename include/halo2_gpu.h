@@ -463,13 +463,16 @@ RustError h2amd_g1_to_lagrange(void* d_out, const void* d_g, uint32_t k, const v
     uint64_t scratch_bytes, void* stream);
 
 /* ======================= tuned MSM window widths (no reference counterpart)
- * widths[i] (0 = the default rule, else 4..20) is the window width of plain MSMs of
- * 2^i .. 2^(i+1) - 1 points, for i < count <= 32; it replaces the whole table. It changes
- * the plain MSM workspace sizes too, so set it before sizing; it is process-global and
- * not synchronised with concurrent MSM calls. h2amd_msm_width(n) reports the width
- * a plain MSM of n points uses. The tuner is halo2_amd/autotune.py. */
+ * widths[i] (0 = the default rule) for i < count <= 64 replaces the whole table: entry
+ * i < 32 (4..20) is the window width of plain MSMs of 2^i .. 2^(i+1) - 1 points, entry
+ * 32 + i (4..22) that of precomputed tables built for 2^i .. 2^(i+1) - 1 bases. It changes
+ * workspace and table sizes too, so set it before sizing; a table is only valid under
+ * the width it was built with (h2amd_msm_table_width(table_n) reports the current one).
+ * Process-global, not synchronised with concurrent MSM calls. h2amd_msm_width(n) reports
+ * the width a plain MSM of n points uses. The tuner is halo2_amd/autotune.py. */
 RustError h2amd_msm_set_widths(const uint8_t* widths, uint32_t count);
 uint32_t h2amd_msm_width(uint64_t n);
+uint32_t h2amd_msm_table_width(uint64_t table_n);
 /* tiles[log_n] (0 = default, else 6..12): log2 of the elements one NTT work-group
  * transforms (radix columns x column group) for transforms of 2^log_n; same rules. */
 RustError h2amd_ntt_set_tiles(const uint8_t* tiles, uint32_t count);

@@ -27,9 +27,19 @@ def test_width_table_host_side():
         for bad in (3, 21):
             with pytest.raises(_lib.Halo2GpuError):
                 set_msm_widths({18: bad})
-        t = np.zeros(33, dtype=np.uint8)
+        t = np.zeros(65, dtype=np.uint8)
         with pytest.raises(_lib.Halo2GpuError):
-            _lib.check(lib.h2amd_msm_set_widths(t.ctypes.data, 33))
+            _lib.check(lib.h2amd_msm_set_widths(t.ctypes.data, 65))
+        # precomputed tables: their own entries (up to 22), the table size follows the width
+        tw0, tb0 = int(lib.h2amd_msm_table_width(n)), int(lib.h2amd_msm_table_bytes(n))
+        set_msm_widths({}, {18: 16})
+        assert int(lib.h2amd_msm_table_width(n)) == 16 and int(lib.h2amd_msm_width(n)) == c0
+        assert int(lib.h2amd_msm_table_bytes(n)) == 64 * n * ((255 + 15) // 16) != tb0
+        set_msm_widths({}, {18: 22})
+        with pytest.raises(_lib.Halo2GpuError):
+            set_msm_widths({}, {18: 23})
+        set_msm_widths({})
+        assert int(lib.h2amd_msm_table_width(n)) == tw0
         with pytest.raises(_lib.Halo2GpuError):
             _lib.check(lib.h2amd_ntt_set_tiles(np.full(5, 13, dtype=np.uint8).ctypes.data, 5))
     finally:

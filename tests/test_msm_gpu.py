@@ -459,3 +459,24 @@ def test_msm_small_scalars_empty_windows(lib, co):
     got = msm.multiexp_batch_device([dv.to_device(c) for c in jobs], pts)
     for j, c in enumerate(jobs):
         assert (got[j] == co.msm(c, hp)).all(), f"job {j}"
+
+
+def test_kzg_table_width_guard(lib, co):
+    """A fixed-base table is only valid under the width it was built with: after the
+    table width changes, ParamsKZG refuses to commit through the old table."""
+    from halo2_amd import autotune, msm
+
+    k = 10
+    params = msm.ParamsKZG(k, 4242)
+    poly = msm.random_fr(1 << k, 4243)
+    ref = params.commit_device(poly)
+    assert (msm.ParamsKZG(k, 4242, fixed_base=False).commit_device(poly) == ref).all()
+    try:
+        autotune.set_msm_widths({}, {k: 7})
+        with pytest.raises(RuntimeError):
+            params.commit_device(poly)
+        fresh = msm.ParamsKZG(k, 4242)  # built under width 7: the same commitment
+        assert (fresh.commit_device(poly) == ref).all()
+        assert (fresh.commit_many_device([poly, poly[: 4 * 100]])[0] == ref).all()
+    finally:
+        autotune.set_msm_widths({})
