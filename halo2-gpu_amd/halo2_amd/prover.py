@@ -267,7 +267,8 @@ def create_proof(params: msm.ParamsKZG, pk: ProvingKey, circuit: Circuit, seed: 
 
     # 3. grand products (permutation/prover.rs:69-273, lookup/prover.rs:294-461)
     beta, gamma = tr.squeeze(), tr.squeeze()
-    ratio = dv.to_device(np.tile(mont_limbs(1), (n, 1)))
+    ratio = dv.empty_u64(4 * n)
+    polyops.poly_fill_one_device(ratio)  # the permutation product multiplies into it
     sc = [dv.to_device(mont_limbs(v).reshape(1, 4)) for v in (beta, gamma, DELTA, dom.omega, 1)]
     ws = int(lib._halo2_permutation_product_device_inputs_workspace_size(n))
     scratch = dv.empty_bytes(ws)
