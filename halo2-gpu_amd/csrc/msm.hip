@@ -54,7 +54,7 @@ constexpr uint32_t HEAVY_THREADS = 256;
 constexpr uint32_t HEAVY_BLOCKS = 512;
 constexpr uint32_t RED_L1 = 16; // buckets per first-level reduction segment
 constexpr uint32_t TREE = 256; // threads (tree leaves) per work-group at later levels
-constexpr uint32_t TREE_K = 8; // pairs folded sequentially into each leaf
+constexpr uint32_t TREE_K = 2; // pairs folded sequentially into each leaf
 constexpr uint32_t MAX_RED_LEVELS = 8;
 constexpr uint32_t MAX_JOBS = 64; // MSMs in one batched pipeline
 
