@@ -50,19 +50,35 @@ __global__ void quotient_lookups_kernel(Fr* __restrict__ values, const Fr* __res
 // Permutation identities (permutation_quotient.h:56-148, the CPU restatement is
 // src/cpu/evaluator.rs:63-147). `delta_start * ext_omega * omega^row` seeds the
 // per-row delta power that advances by delta per column across all sets.
-__global__ void quotient_permutation_kernel(Fr* __restrict__ values, const Fr* const* __restrict__ zs,
-    const Fr* const* __restrict__ perms, const Fr* const* __restrict__ cols, const Fr* __restrict__ l0,
-    const Fr* __restrict__ l_last, const Fr* __restrict__ l_active, Fr beta, Fr gamma, Fr y, Fr delta, Fr delta_start,
-    Fr cur_ext_omega, Fr omega, uint64_t n_sets, uint64_t chunk_len, uint64_t n_cols, int32_t last_rotation,
-    int32_t rot_scale, int32_t isize, uint64_t n)
+// omega^row for the first row of each thread: omega^(block base) from one
+// exponentiation per work-group times omega^threadIdx.x from an LDS table
+// built by doubling (8 dependent products), instead of an exponentiation per
+// thread; the per-thread stride power is shared the same way.
+__global__ void __launch_bounds__(kBlock) quotient_permutation_kernel(Fr* __restrict__ values,
+    const Fr* const* __restrict__ zs, const Fr* const* __restrict__ perms, const Fr* const* __restrict__ cols,
+    const Fr* __restrict__ l0, const Fr* __restrict__ l_last, const Fr* __restrict__ l_active, Fr beta, Fr gamma,
+    Fr y, Fr delta, Fr delta_start, Fr cur_ext_omega, Fr omega, uint64_t n_sets, uint64_t chunk_len,
+    uint64_t n_cols, int32_t last_rotation, int32_t rot_scale, int32_t isize, uint64_t n)
 {
+    __shared__ Fr pw[kBlock + 1];
+    __shared__ Fr s_base, s_step;
     const Fr one = Fr::one();
     const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    if (tid >= n)
-        return;
-    Fr beta_term = cur_ext_omega * Fr::pow_u64(omega, tid);
-    const Fr wstep = Fr::pow_u64(omega, stride);
+    if (threadIdx.x == 0) {
+        pw[0] = one;
+        pw[1] = omega;
+        s_base = cur_ext_omega * Fr::pow_u64(omega, (uint64_t)blockIdx.x * blockDim.x);
+        s_step = Fr::pow_u64(omega, stride);
+    }
+    __syncthreads();
+    for (unsigned h = 1; h < kBlock; h <<= 1) { // pw[h + 1 .. 2h] from pw[1 .. h] and pw[h]
+        if (threadIdx.x < h)
+            pw[h + 1 + threadIdx.x] = pw[1 + threadIdx.x] * pw[h];
+        __syncthreads();
+    }
+    Fr beta_term = s_base * pw[threadIdx.x];
+    const Fr wstep = s_step;
     for (uint64_t i = tid; i < n; i += stride) {
         const uint64_t rn = rot(i, (int64_t)rot_scale, isize);
         const uint64_t rl = rot(i, (int64_t)last_rotation * rot_scale, isize);
