@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--proof-k", type=int, default=20, help="k of the standard-gate proof measured under extra")
     ap.add_argument("--proof-reps", type=int, default=10)
     ap.add_argument("--no-proof", action="store_true", help="skip the end-to-end proof measurement")
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="length of the Poisson-arrival serving run (0 skips it)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one plain device copy of this size first (FETCH_SIZE/WRITE_SIZE calibration)")
     return ap.parse_args()
@@ -246,6 +248,58 @@ def proof_measurements(msm, torch, k: int, reps: int) -> dict:
     return out
 
 
+def serving_measurements(msm, torch, seconds: float, rate: float = 4.0, ks=(16, 18, 20, 22),
+                         weights=(0.4, 0.3, 0.2, 0.1), seed: int = 55) -> dict:
+    """A proving service on one GPU: requests arrive as a Poisson process of `rate` per
+    second for `seconds`, each for the standard-gate circuit at k drawn with `weights`;
+    one server proves them first come first served with every k's proving key, SRS and
+    fixed-base tables resident. Each k has two pre-generated witnesses used in turn
+    (fresh blinders every proof). Reports proofs/s over the run and the latency from
+    arrival to finished proof (queueing included)."""
+    import dataclasses
+
+    import numpy as np
+
+    from halo2_amd import prover
+
+    setups = {}
+    for k in ks:
+        c = prover.synthesize_standard(k, seed + k)
+        pk = prover.keygen(c)
+        params = msm.ParamsKZG(k, 0x5EED + k)
+        prover.create_proof(params, pk, c, seed=1)  # warm-up: g_lagrange, tables, key parts
+        witnesses = [c, dataclasses.replace(c, advice=prover.fresh_witness(c, seed + 100 + k))]
+        setups[k] = (params, pk, witnesses)
+    rng = np.random.default_rng(seed)
+    arrivals, t = [], 0.0
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t > seconds:
+            break
+        arrivals.append((t, int(rng.choice(ks, p=weights))))
+    lat, per_k, ok, done = [], {k: [] for k in ks}, True, 0.0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i, (ta, k) in enumerate(arrivals):
+        wait = ta - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        params, pk, wits = setups[k]
+        r = prover.create_proof(params, pk, wits[i % 2], seed=1000 + i)
+        torch.cuda.current_stream().synchronize()
+        done = time.perf_counter() - t0
+        lat.append((done - ta) * 1e3)
+        per_k[k].append((done - ta) * 1e3)
+        ok &= all(r["checks"].values())
+    return {"rate_per_s": rate, "seconds": seconds, "ks": list(ks), "weights": list(weights),
+            "requests": len(arrivals), "proofs_per_s": round(len(arrivals) / max(done, seconds), 3),
+            "p50_ms": round(float(np.percentile(lat, 50)), 2) if lat else None,
+            "p99_ms": round(float(np.percentile(lat, 99)), 2) if lat else None,
+            "by_k": {str(k): {"n": len(v), "p50_ms": round(float(np.percentile(v, 50)), 2) if v else None}
+                     for k, v in per_k.items()},
+            "all_checks_hold": bool(ok)}
+
+
 def proofs_in_flight(prover, params, pk, circuit, torch, streams: int, per_stream: int) -> dict:
     """Serving mode: `streams` host threads, each proving on its own HIP stream, so one
     proof's latency-bound kernels (bucket-reduction trees, scans) overlap another's
@@ -440,6 +494,10 @@ def main() -> int:
     # ------------------------- end-to-end proof of the standard-gate circuit
     if not args.no_proof:
         extra[f"proof_k{args.proof_k}"] = proof_measurements(msm, torch, args.proof_k, args.proof_reps)
+
+    # ---------------- proving service: Poisson arrivals over k in {16..22}
+    if not args.no_proof and args.serve_seconds > 0:
+        extra["serving"] = serving_measurements(msm, torch, args.serve_seconds)
 
     # ------------------ tuner: MSM width / NTT tile per size vs the default rules
     if not args.no_autotune:

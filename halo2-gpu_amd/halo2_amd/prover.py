@@ -115,10 +115,7 @@ def synthesize_standard(k: int, seed: int, table_bits: int = 16) -> Circuit:
     tv = np.zeros(n, dtype=np.uint64)
     tv[:table_size] = np.arange(table_size, dtype=np.uint64)
     table = _mont_column(tv)
-    # a, b with random copy cycles: every cell of a cycle takes its first cell's value
-    a = msm.random_fr(n, seed * 16 + 5)
-    b = msm.random_fr(n, seed * 16 + 6)
-    ah, bh = dv.to_host(a), dv.to_host(b)
+    # random copy cycles of 2..8 cells over a and b (column 0 = a, 1 = b)
     cells = rng.permutation(2 * u)[: max(2, (2 * u) // 8)]
     cycles, i = [], 0
     while i + 2 <= len(cells):
@@ -127,12 +124,30 @@ def synthesize_standard(k: int, seed: int, table_bits: int = 16) -> Circuit:
         if len(cyc) >= 2:
             cycles.append(cyc)
         i += ln
-    cols = [ah, bh]
-    for cyc in cycles:
-        v = cols[cyc[0][0]][cyc[0][1]].copy()
-        for c, r in cyc[1:]:
-            cols[c][r] = v
-    a, b = dv.to_device(ah), dv.to_device(bh)
+    instance = [torch.zeros(4 * n, dtype=torch.int64, device=dv.device())]
+    circuit = Circuit(k, [q_a, q_b, q_m, q_c, table], [], instance, cycles, u)
+    circuit.advice = fresh_witness(circuit, seed, table_size)
+    return circuit
+
+
+def fresh_witness(circuit: Circuit, seed: int, table_size: int | None = None) -> list:
+    """A new satisfying assignment of the advice columns for the same circuit (fixed
+    columns, copy cycles and table unchanged, so the proving key stays valid): a, b
+    random with every cell of a cycle equal to its first cell, c = q_a a + q_b b + q_m a b,
+    d drawn from the table, e free, random blinders on the last rows."""
+    k, u = circuit.k, circuit.usable
+    n = 1 << k
+    if table_size is None:
+        table_size = min(1 << 16, u)
+    rng = np.random.default_rng(seed + 0x5EED)
+    q_a, q_b, q_m = circuit.fixed[:3]
+    a = msm.random_fr(n, seed * 16 + 5)
+    b = msm.random_fr(n, seed * 16 + 6)
+    flat = np.concatenate([dv.to_host(a), dv.to_host(b)])  # cell (col, row) at col * n + row
+    src = np.fromiter((cyc[0][0] * n + cyc[0][1] for cyc in circuit.cycles for _ in cyc[1:]), dtype=np.int64)
+    dst = np.fromiter((c * n + r for cyc in circuit.cycles for c, r in cyc[1:]), dtype=np.int64)
+    flat[dst] = flat[src]  # sources are first cells, never destinations
+    a, b = dv.to_device(flat[:n]), dv.to_device(flat[n:])
     # c = q_a a + q_b b + q_m a b (q_c = -1); zero selectors leave c free on the blinding rows
     c = dv.empty_u64(4 * n)
     t = dv.empty_u64(4 * n)
@@ -146,8 +161,7 @@ def synthesize_standard(k: int, seed: int, table_bits: int = 16) -> Circuit:
     e = msm.random_fr(n, seed * 16 + 7)
     for j, col in enumerate((a, b, c, d)):  # blinders
         _set_rows(col, u, msm.random_fr(n - u, seed * 16 + 8 + j))
-    instance = [torch.zeros(4 * n, dtype=torch.int64, device=dv.device())]
-    return Circuit(k, [q_a, q_b, q_m, q_c, table], [a, b, c, d, e], instance, cycles, u)
+    return [a, b, c, d, e]
 
 
 # ---------------------------------------------------------------- proving key

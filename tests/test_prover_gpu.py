@@ -49,3 +49,20 @@ def test_proof_pipeline_detects_bad_witness(lib):
     assert out["checks"]["permutation_product_closes"] and not out["checks"]["h_is_a_polynomial"]
     out = _prove(10, 4, break_lookup)
     assert not out["checks"]["lookup_product_closes"] and not out["checks"]["h_is_a_polynomial"]
+
+
+def test_fresh_witness_same_key(lib):
+    """A fresh witness for the same circuit proves under the same proving key; the
+    proof differs."""
+    import dataclasses
+
+    from halo2_amd import fields as F, msm, prover
+
+    circuit = prover.synthesize_standard(10, 6, table_bits=8)
+    pk = prover.keygen(circuit)
+    params = msm.ParamsKZG(10, 77 % F.R_MOD)
+    first = prover.create_proof(params, pk, circuit, seed=1)
+    other = dataclasses.replace(circuit, advice=prover.fresh_witness(circuit, 7, 1 << 8))
+    second = prover.create_proof(params, pk, other, seed=1)
+    assert all(first["checks"].values()) and all(second["checks"].values())
+    assert first["proof"] != second["proof"]
