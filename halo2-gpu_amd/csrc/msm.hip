@@ -173,8 +173,9 @@ MsmPlan plan_core(uint64_t n, uint32_t c, uint32_t J, uint64_t table_n, uint32_t
     p.ntiles = (uint32_t)((n + R1_TILE - 1) / R1_TILE);
     const uint64_t work = n * p.W;
     p.max_chunks = work / R2_CHUNK + p.NC;
-    // ~2^21 accumulation threads, 16..128 entries each
-    uint64_t s = work >> 21;
+    // ~2^20 accumulation threads, 16..128 entries each: longer runs than buckets
+    // split fewer buckets between threads (less merging) and still fill the chip
+    uint64_t s = work >> 20;
     if (s < 16)
         s = 16;
     if (s > 128)
