@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import ctypes
 import hashlib
+import threading
 import time
 from dataclasses import dataclass, field
 
@@ -238,6 +239,17 @@ def _grand_product_z(ratio: torch.Tensor, n: int, seed: int) -> torch.Tensor:
     return z
 
 
+_side = threading.local()
+
+
+def _side_stream() -> torch.cuda.Stream:
+    """A second stream per host thread for transforms that overlap a commitment MSM
+    (its sort, bucket and reduction kernels leave the device partly idle)."""
+    if getattr(_side, "stream", None) is None:
+        _side.stream = torch.cuda.Stream()
+    return _side.stream
+
+
 def _check_closing(z: torch.Tensor, u: int) -> bool:
     return bool((z.view(-1, 4)[u].cpu().numpy().view(np.uint64) == mont_limbs(1)).all())
 
@@ -261,10 +273,15 @@ def create_proof(params: msm.ParamsKZG, pk: ProvingKey, circuit: Circuit, seed: 
     def commit_lagrange_many(cols):
         return params.commit_lagrange_many_device(cols)
 
-    # 1. advice (prover.rs:282-466): commitments in Lagrange form, then coefficients
+    main, side = torch.cuda.current_stream(), _side_stream()
+    # 1. advice (prover.rs:282-466): commitments in Lagrange form; the coefficient forms
+    #    are computed on the side stream meanwhile
+    side.wait_stream(main)
+    with torch.cuda.stream(side):
+        advice_polys = dom.lagrange_to_coeff_many_device_inputs(circuit.advice)
     for pt in commit_lagrange_many(circuit.advice):
         tr.write_point(pt)
-    advice_polys = dom.lagrange_to_coeff_many_device_inputs(circuit.advice)
+    main.wait_stream(side)
     phase("advice")
 
     # 2. lookup commit_permuted (lookup/prover.rs:552-627): theta-compression of one
@@ -295,10 +312,13 @@ def create_proof(params: msm.ParamsKZG, pk: ProvingKey, circuit: Circuit, seed: 
     z_lookup = _grand_product_z(lratio, n, seed * 7 + 4)
     checks["permutation_product_closes"] = _check_closing(z_perm, u)
     checks["lookup_product_closes"] = _check_closing(z_lookup, u)
+    side.wait_stream(main)
+    with torch.cuda.stream(side):
+        z_perm_poly, z_lookup_poly, p_in_poly, p_tab_poly = dom.lagrange_to_coeff_many_device_inputs(
+            [z_perm, z_lookup, p_in, p_tab])
     for pt in commit_lagrange_many([z_perm, z_lookup]):
         tr.write_point(pt)
-    z_perm_poly, z_lookup_poly, p_in_poly, p_tab_poly = dom.lagrange_to_coeff_many_device_inputs(
-        [z_perm, z_lookup, p_in, p_tab])
+    main.wait_stream(side)
     phase("grand_products")
 
     # 4. vanishing argument's random polynomial (vanishing/prover.rs:68-120)
