@@ -244,7 +244,9 @@ def proof_measurements(msm, torch, k: int, reps: int) -> dict:
            "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
            "phases_ms": runs[med]["times_ms"], "proof_bytes": len(runs[med]["proof"]),
            "all_checks_hold": all(all(r["checks"].values()) for r in runs)}
-    out["in_flight_2"] = proofs_in_flight(prover, params, pk, circuit, torch, 2, max(2, reps // 2))
+    for streams in (2, 4):
+        out[f"in_flight_{streams}"] = proofs_in_flight(prover, params, pk, circuit, torch, streams,
+                                                       max(2, reps // 2))
     return out
 
 
