@@ -312,18 +312,22 @@ def create_proof(params: msm.ParamsKZG, pk: ProvingKey, circuit: Circuit, seed: 
     z_lookup = _grand_product_z(lratio, n, seed * 7 + 4)
     checks["permutation_product_closes"] = _check_closing(z_perm, u)
     checks["lookup_product_closes"] = _check_closing(z_lookup, u)
+    # 4. the vanishing argument's random polynomial (vanishing/prover.rs:68-120) depends
+    #    on no challenge: drawn as n uniform Lagrange values (a uniform polynomial of
+    #    degree < n, as halo2's uniform coefficients), committed in the same batch as the
+    #    grand products against g_lagrange; its transcript point still follows theirs
+    random_values = msm.random_fr(n, seed * 7 + 5)
     side.wait_stream(main)
     with torch.cuda.stream(side):
-        z_perm_poly, z_lookup_poly, p_in_poly, p_tab_poly = dom.lagrange_to_coeff_many_device_inputs(
-            [z_perm, z_lookup, p_in, p_tab])
-    for pt in commit_lagrange_many([z_perm, z_lookup]):
-        tr.write_point(pt)
+        z_perm_poly, z_lookup_poly, p_in_poly, p_tab_poly, random_poly = dom.lagrange_to_coeff_many_device_inputs(
+            [z_perm, z_lookup, p_in, p_tab, random_values])
+    z_perm_pt, z_lookup_pt, random_pt = commit_lagrange_many([z_perm, z_lookup, random_values])
+    tr.write_point(z_perm_pt)
+    tr.write_point(z_lookup_pt)
     main.wait_stream(side)
     phase("grand_products")
 
-    # 4. vanishing argument's random polynomial (vanishing/prover.rs:68-120)
-    random_poly = msm.random_fr(n, seed * 7 + 5)
-    tr.write_point(params.commit_device(random_poly))
+    tr.write_point(random_pt)
     y = tr.squeeze()
     phase("vanishing_commit")
 
