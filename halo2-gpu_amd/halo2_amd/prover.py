@@ -358,12 +358,17 @@ def create_proof(params: msm.ParamsKZG, pk: ProvingKey, circuit: Circuit, seed: 
         tr.write_scalar(int.from_bytes(np.ascontiguousarray(e).tobytes(), "little") * pow(1 << 256, -1, R_MOD) % R_MOD)
     phase("evaluations")
 
-    # 7. SHPLONK multi-open (shplonk/prover.rs:97-441), challenges from the transcript
-    sy, sv, su = tr.squeeze(), tr.squeeze(), tr.squeeze()
+    # 7. SHPLONK multi-open (shplonk/prover.rs:97-441): y and v squeezed first, h written,
+    #    then u squeezed from the transcript that holds h, then h_final written
+    sy, sv = tr.squeeze(), tr.squeeze()
     ids = {}
     pq = [ProverQuery(pt, p, ids.setdefault(id(p), len(ids))) for p, pt in queries]
-    out = create_proof_device(params, pq, sy, sv, su)
-    tr.write_point(out["h"])
+
+    def u_after_h(h):
+        tr.write_point(h)
+        return tr.squeeze()
+
+    out = create_proof_device(params, pq, sy, sv, u_after_h)
     tr.write_point(out["h_final"])
     phase("opening")
     return {"proof": bytes(tr.proof), "times_ms": times, "checks": checks}

@@ -128,10 +128,12 @@ def _zeros(n: int) -> torch.Tensor:
     return torch.zeros(4 * n, dtype=torch.int64, device=dv.device())
 
 
-def create_proof_device(params, queries: list[ProverQuery], y: int, v: int, u: int) -> dict:
+def create_proof_device(params, queries: list[ProverQuery], y: int, v: int, u) -> dict:
     """prover.rs:97-441 with the transcript's challenges given: returns the two commitments
     ("h", "h_final", 96-byte projective points as (12,) u64) and the device polynomials
-    h_x (n coefficients) and h_x_final (n - 1 coefficients)."""
+    h_x (n coefficients) and h_x_final (n - 1 coefficients). `u` is the challenge or, as
+    in the protocol (u is squeezed after h is written, prover.rs:388-392), a function
+    that takes h's commitment and returns u."""
     n = params.n
     rotation_sets, super_points = construct_intermediate_sets(queries)
     for rs in rotation_sets:
@@ -167,6 +169,8 @@ def create_proof_device(params, queries: list[ProverQuery], y: int, v: int, u: i
     for j, q in enumerate(quotient_polys):
         polyops.poly_multiply_add_device_at_lut_offset(d_h_x, q, d_v_powers, j)
     h = params.commit_device(d_h_x)
+    if callable(u):
+        u = u(h) % R_MOD
 
     lin, z_diffs = [], []
     for rs, d_l_x in zip(rotation_sets, p_xs):  # linearisation_contribution, prover.rs:331-386

@@ -32,6 +32,10 @@ def test_shplonk_create_proof(lib, co):
     queries = [ProverQuery(p, d_polys[i], i) for i, pts in plan for p in pts]
     y, v, u = (rng.randrange(R) for _ in range(3))
     out = create_proof_device(params, queries, y, v, u)
+    # u squeezed after h is written (a function of h's commitment) gives the same proof
+    seen = []
+    again = create_proof_device(params, queries, y, v, lambda h: seen.append(h.copy()) or u)
+    assert len(seen) == 1 and (seen[0] == out["h"]).all() and (again["h_final"] == out["h_final"]).all()
 
     # the same steps on the oracle
     val = lambda row: F.value_of(row)  # noqa: E731
