@@ -279,6 +279,7 @@ def create_proof(params: msm.ParamsKZG, pk: ProvingKey, circuit: Circuit, seed: 
     side.wait_stream(main)
     with torch.cuda.stream(side):
         advice_polys = dom.lagrange_to_coeff_many_device_inputs(circuit.advice)
+        instance_polys = dom.lagrange_to_coeff_many_device_inputs(circuit.instance)
     for pt in commit_lagrange_many(circuit.advice):
         tr.write_point(pt)
     main.wait_stream(side)
@@ -333,7 +334,7 @@ def create_proof(params: msm.ParamsKZG, pk: ProvingKey, circuit: Circuit, seed: 
 
     # 5. h(X) (evaluation.rs:1056-1417), divided by X^n - 1, in pieces of n
     num = evaluate_h_device(
-        dom, pk.gates, pk.fixed_polys, pk.l0, pk.l_last, pk.l_active, advice_polys, circuit.instance, [], y, beta,
+        dom, pk.gates, pk.fixed_polys, pk.l0, pk.l_last, pk.l_active, advice_polys, instance_polys, [], y, beta,
         gamma, theta, lookups=[LookupCommitted(pk.lookup, z_lookup_poly, p_in, p_tab, p_in_poly, p_tab_poly)],
         permutation=PermutationCommitted([z_perm_poly], [("advice", 0), ("advice", 1)], pk.sigma_polys),
         blinding_factors=BLINDING_FACTORS, cs_degree=CS_DEGREE, key_parts=pk.key_parts)
@@ -347,15 +348,24 @@ def create_proof(params: msm.ParamsKZG, pk: ProvingKey, circuit: Circuit, seed: 
         tr.write_point(pt)
     phase("quotient")
 
-    # 6. evaluations at x (prover.rs:513-592)
+    # 6. evaluations at x (prover.rs:513-592), in halo2's order: instance, advice, fixed,
+    #    the vanishing random polynomial, sigma, permutation products at x and omega x,
+    #    lookup product at x and omega x, permuted input at x and omega^-1 x, permuted
+    #    table at x. h is opened as h_0 + x^n h_1 + x^2n h_2 (vanishing/prover.rs:
+    #    Constructed::evaluate); its value is not written, the verifier derives it.
     x = tr.squeeze()
     xw, xwi = x * dom.omega % R_MOD, x * dom.omega_inv % R_MOD
-    queries = [(p, x) for p in advice_polys + pk.fixed_polys + pk.sigma_polys + pieces + [random_poly]]
+    queries = [(p, x) for p in instance_polys + advice_polys + pk.fixed_polys + [random_poly] + pk.sigma_polys]
     queries += [(z_perm_poly, x), (z_perm_poly, xw), (z_lookup_poly, x), (z_lookup_poly, xw), (p_in_poly, x),
                 (p_in_poly, xwi), (p_tab_poly, x)]
     evals = polyops.batch_eval_polynomial_d2h([p for p, _ in queries], [pt for _, pt in queries])
     for e in evals:
         tr.write_scalar(int.from_bytes(np.ascontiguousarray(e).tobytes(), "little") * pow(1 << 256, -1, R_MOD) % R_MOD)
+    xn = pow(x, n, R_MOD)
+    h_poly = pieces[0].clone()
+    for i in range(1, len(pieces)):
+        polyops.poly_multiply_add_device(h_poly, pieces[i], pow(xn, i, R_MOD))
+    queries.append((h_poly, x))
     phase("evaluations")
 
     # 7. SHPLONK multi-open (shplonk/prover.rs:97-441): y and v squeezed first, h written,
