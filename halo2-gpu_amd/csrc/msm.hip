@@ -52,7 +52,7 @@ constexpr uint32_t R2_CHUNK = 8192; // entries per round-2 chunk (staged in LDS)
 constexpr uint32_t LIGHT = 32; // pieces of a split bucket summed by one thread
 constexpr uint32_t HEAVY_THREADS = 256;
 constexpr uint32_t HEAVY_BLOCKS = 512;
-constexpr uint32_t RED_L1 = 16; // buckets per first-level reduction segment
+constexpr uint32_t RED_L1 = 16; // buckets per first-level reduction segment (8 up to 2^16 buckets per window)
 constexpr uint32_t TREE = 256; // threads (tree leaves) per work-group at later levels
 constexpr uint32_t TREE_K = 2; // pairs folded sequentially into each leaf
 constexpr uint32_t MAX_RED_LEVELS = 8;
@@ -182,7 +182,10 @@ MsmPlan plan_core(uint64_t n, uint32_t c, uint32_t J, uint64_t table_n, uint32_t
         s = 128;
     p.S = (uint32_t)s;
     p.nthr = (work + p.S - 1) / p.S;
-    p.L1 = p.B < RED_L1 ? p.B : RED_L1;
+    // shorter first-level runs shorten the serial chain where the reduction's
+    // latency dominates (few buckets); with many buckets the extra tree work wins
+    const uint32_t l1 = p.B <= (1u << 16) ? RED_L1 / 2 : RED_L1;
+    p.L1 = p.B < l1 ? p.B : l1;
     p.Mlev[0] = p.B / p.L1;
     uint32_t lev = 0;
     while (p.Mlev[lev] > 1) {
