@@ -24,8 +24,14 @@ def empty_bytes(n_bytes: int) -> torch.Tensor:
 
 
 def to_device(a: np.ndarray) -> torch.Tensor:
+    """Host words to a new device tensor, stream-ordered and without waiting: the words
+    are staged in pinned memory (torch's caching host allocator keeps the staging block
+    until the copy has run), so the host does not drain the stream as a pageable copy
+    would -- the small tables and scalars of every launch sequence stay off the
+    critical path."""
     a = np.ascontiguousarray(a, dtype=np.uint64)
-    return torch.from_numpy(a.view(np.int64).reshape(-1).copy()).to(device())
+    staged = torch.from_numpy(a.view(np.int64).reshape(-1)).pin_memory()
+    return staged.to(device(), non_blocking=True)
 
 
 def to_host(t: torch.Tensor, cols: int = 4) -> np.ndarray:

@@ -61,7 +61,7 @@ def _host_scalar(v: int) -> np.ndarray:
 
 
 def _ptr_table(tensors) -> torch.Tensor:
-    return torch.tensor([dv.ptr(t) for t in tensors] or [0], dtype=torch.int64, device=dv.device())
+    return dv.to_device(np.array([dv.ptr(t) for t in tensors] or [0], dtype=np.uint64))
 
 
 def lagrange_to_extend_part(domain, poly: torch.Tensor, extended_omega_factor: int) -> torch.Tensor:

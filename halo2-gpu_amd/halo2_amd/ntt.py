@@ -165,7 +165,7 @@ def divide_by_vanishing_poly_device(d_poly: torch.Tensor, d_t_evals: torch.Tenso
 def extended_from_lagrange_vec_device(d_out: torch.Tensor, d_parts: list[torch.Tensor], n: int) -> None:
     """src/cuda/funcs/ntt.rs:80-110 (pointer table uploaded to the device)."""
     lib = _lib.load()
-    table = torch.tensor([dv.ptr(p) for p in d_parts], dtype=torch.int64, device=dv.device())
+    table = dv.to_device(np.array([dv.ptr(p) for p in d_parts], dtype=np.uint64))
     _lib.check(lib._halo2_extended_from_lagrange_vec_device(dv.ptr(d_out), dv.ptr(table), len(d_parts), n,
                                                             dv.stream()),
                "_halo2_extended_from_lagrange_vec_device")
