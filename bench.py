@@ -9,7 +9,7 @@ twiddle-table build + two LDS-staged Stockham passes, uniform seeded Fr input.
 MSM throughput at 2^20, 2^22, 2^24 against a seeded-tau SRS is reported under
 `extra`.
 
-Prints ONE JSON line. `--gpus N` with N > 1 exits: this build runs on one GPU.
+Prints ONE JSON line. `--gpus N` with N > 1 exits: the library runs on one GPU.
 """
 from __future__ import annotations
 
@@ -344,7 +344,7 @@ def main() -> int:
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}): MSM chunks, NTT batches and "
-              f"quotient parts would shard across GPUs without cross-GPU kernels, which this build "
+              f"quotient parts would shard across GPUs without cross-GPU kernels, which this library "
               f"does not implement", file=sys.stderr)
         return 2
 

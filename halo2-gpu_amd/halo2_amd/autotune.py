@@ -1,6 +1,5 @@
 """Per-size tuning of the MSM window width and the NTT work-group tile, by
-measurement on the device (the north star's "window width and NTT radix/tile
-picked per size by an autotuner").
+measurement on the device (window width and NTT tile picked per size).
 
 The library keeps two process-global tables (h2amd_msm_set_widths,
 h2amd_ntt_set_tiles); every choice they hold gives the same results, only the

@@ -4,7 +4,7 @@ commitments, lookup permutation (commit_permuted), permutation and lookup grand
 products, the vanishing argument's random polynomial, h(X) over the extended
 domain (evaluate_h), its pieces, the evaluations at x and the SHPLONK opening.
 
-The circuit is the standard-gate circuit of the north star's k = 20 proof:
+The circuit is the standard-gate circuit of the k = 20 proving benchmark:
 5 advice columns (a, b, c = q_a a + q_b b + q_m a b, a lookup input d, a free e),
 fixed selectors q_a, q_b, q_m, q_c = -1 and a 16-bit table, one instance column,
 copy constraints in random cycles of 2..8 cells over a and b, and the lookup
