@@ -117,8 +117,8 @@ uint32_t default_width(uint64_t n, bool shared, uint32_t bits)
     int c = (int)log2_floor(n < 1 ? 1 : n) - (shared ? 0 : 4);
     if (c < 4)
         c = 4;
-    if (c > (shared ? 22 : 20))
-        c = shared ? 22 : 20;
+    if (c > 20) // shared buckets too: measured 2^22 tables 7.6 ms at 20 bits vs 8.3 at 22, 2^24 equal
+        c = 20;
     // same window count, narrowest width: the top window is as full as the others
     const uint32_t w0 = (bits + (uint32_t)c - 1) / (uint32_t)c;
     return (bits + w0 - 1) / w0;
