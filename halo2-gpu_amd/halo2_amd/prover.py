@@ -311,8 +311,6 @@ def create_proof(params: msm.ParamsKZG, pk: ProvingKey, circuit: Circuit, seed: 
     z_perm = _grand_product_z(ratio, n, seed * 7 + 3)
     lratio = lookup_product_device(p_in, p_tab, inp, tab, beta, gamma)
     z_lookup = _grand_product_z(lratio, n, seed * 7 + 4)
-    checks["permutation_product_closes"] = _check_closing(z_perm, u)
-    checks["lookup_product_closes"] = _check_closing(z_lookup, u)
     # 4. the vanishing argument's random polynomial (vanishing/prover.rs:68-120) depends
     #    on no challenge: drawn as n uniform Lagrange values (a uniform polynomial of
     #    degree < n, as halo2's uniform coefficients), committed in the same batch as the
@@ -323,6 +321,9 @@ def create_proof(params: msm.ParamsKZG, pk: ProvingKey, circuit: Circuit, seed: 
         z_perm_poly, z_lookup_poly, p_in_poly, p_tab_poly, random_poly = dom.lagrange_to_coeff_many_device_inputs(
             [z_perm, z_lookup, p_in, p_tab, random_values])
     z_perm_pt, z_lookup_pt, random_pt = commit_lagrange_many([z_perm, z_lookup, random_values])
+    # read back after the commitment (which waits for the stream anyway), not before it
+    checks["permutation_product_closes"] = _check_closing(z_perm, u)
+    checks["lookup_product_closes"] = _check_closing(z_lookup, u)
     tr.write_point(z_perm_pt)
     tr.write_point(z_lookup_pt)
     main.wait_stream(side)
@@ -342,10 +343,11 @@ def create_proof(params: msm.ParamsKZG, pk: ProvingKey, circuit: Circuit, seed: 
     ntt.fft_normal_device(ntt.IFFT, dom.extended_k, num, num, dom.extended_omega_inv, dom.extended_ifft_divisor)
     ntt.distribute_powers_zeta_device(num, [dom.g_coset_inv, dom.g_coset])
     top = num.view(-1, 4)[(CS_DEGREE - 1) * n:]
-    checks["h_is_a_polynomial"] = not bool(top.any().item())
+    nonzero_top = top.any()  # read back after the commitment below
     pieces = [num[4 * i * n: 4 * (i + 1) * n] for i in range(CS_DEGREE - 1)]
     for pt in params.commit_many_device(pieces):
         tr.write_point(pt)
+    checks["h_is_a_polynomial"] = not bool(nonzero_top.item())
     phase("quotient")
 
     # 6. evaluations at x (prover.rs:513-592), in halo2's order: instance, advice, fixed,
