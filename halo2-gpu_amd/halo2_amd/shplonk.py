@@ -12,6 +12,7 @@ CPU.
 """
 from __future__ import annotations
 
+import threading
 from dataclasses import dataclass
 
 import numpy as np
@@ -44,6 +45,19 @@ class _Commitment:
 class RotationSet:
     points: list[int]
     commitments: list[_Commitment]
+
+
+_side = threading.local()
+
+
+def _side_streams(count: int) -> list:
+    """Up to `count` side streams of this host thread (created once)."""
+    pool = getattr(_side, "pool", None)
+    if pool is None:
+        pool = _side.pool = []
+    while len(pool) < count:
+        pool.append(torch.cuda.Stream())
+    return pool[:count]
 
 
 def powers(x: int, count: int) -> list[int]:
@@ -143,27 +157,39 @@ def create_proof_device(params, queries: list[ProverQuery], y: int, v: int, u) -
     d_y_powers = _dev_table(powers(y, max_c))
 
     quotient_polys, p_xs = [], []
-    for rs in rotation_sets:  # quotient_contribution, prover.rs:183-268
-        d_p_x = _zeros(n)
-        for i, c in enumerate(rs.commitments):
-            d_in = c.poly if c.poly.numel() == 4 * n else torch.cat([c.poly, _zeros(n - c.poly.numel() // 4)])
-            polyops.poly_multiply_add_device_at_lut_offset(d_p_x, d_in, d_y_powers, i)
-        r_x = [0] * max(len(c.low_degree_equivalent) for c in rs.commitments)
-        for c, py in zip(rs.commitments, powers(y, len(rs.commitments))):
-            for i, coef in enumerate(c.low_degree_equivalent):
-                r_x[i] = (r_x[i] + coef * py) % R_MOD
-        d_n_x = dv.empty_u64(4 * n)
-        polyops.poly_sub_short_out_of_place_device(d_n_x, d_p_x, _dev_table(r_x))
-        d_q = d_n_x
-        for i, point in enumerate(rs.points):
-            d_root = _dev_scalar(point)
-            if i + 1 == len(rs.points):
-                d_q = polyops.kate_division_device_padded_with_d_root(d_q, d_root, n)
-            else:
-                d_q = polyops.kate_division_device_with_d_root(d_q, d_root)
-        quotient_polys.append(d_q)
-        p_xs.append(d_p_x)
+    # the rotation sets are independent until h is combined: each set's folds and
+    # divisions (latency-bound chains of small launches) run on its own stream
+    main = torch.cuda.current_stream()
+    streams = [main] + _side_streams(min(len(rotation_sets), 3) - 1)
+    for st in streams[1:]:
+        st.wait_stream(main)
+    for si, rs in enumerate(rotation_sets):  # quotient_contribution, prover.rs:183-268
+        st = streams[si % len(streams)]
+        with torch.cuda.stream(st):
+            d_p_x = _zeros(n)
+            for i, c in enumerate(rs.commitments):
+                d_in = c.poly if c.poly.numel() == 4 * n else torch.cat([c.poly, _zeros(n - c.poly.numel() // 4)])
+                polyops.poly_multiply_add_device_at_lut_offset(d_p_x, d_in, d_y_powers, i)
+            r_x = [0] * max(len(c.low_degree_equivalent) for c in rs.commitments)
+            for c, py in zip(rs.commitments, powers(y, len(rs.commitments))):
+                for i, coef in enumerate(c.low_degree_equivalent):
+                    r_x[i] = (r_x[i] + coef * py) % R_MOD
+            d_n_x = dv.empty_u64(4 * n)
+            polyops.poly_sub_short_out_of_place_device(d_n_x, d_p_x, _dev_table(r_x))
+            d_q = d_n_x
+            for i, point in enumerate(rs.points):
+                d_root = _dev_scalar(point)
+                if i + 1 == len(rs.points):
+                    d_q = polyops.kate_division_device_padded_with_d_root(d_q, d_root, n)
+                else:
+                    d_q = polyops.kate_division_device_with_d_root(d_q, d_root)
+            quotient_polys.append(d_q)
+            p_xs.append(d_p_x)
 
+    for st in streams[1:]:
+        main.wait_stream(st)
+    for t in quotient_polys + p_xs:  # possibly made on a side stream, used on this one from here
+        t.record_stream(main)
     d_v_powers = _dev_table(powers(v, max(1, len(quotient_polys))))
     d_h_x = _zeros(n)
     for j, q in enumerate(quotient_polys):
