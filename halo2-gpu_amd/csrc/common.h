@@ -86,6 +86,12 @@ struct WorkspaceSizer {
     void take(uint64_t n, uint64_t align = 256) { bytes = align_up(bytes, align) + n; }
 };
 
+// Copies a (na bytes) to d_a and b (nb bytes) to d_b on `stream` through a ring of
+// pinned host buffers, so the caller's host arrays may go away on return without a
+// stream synchronisation (staging.hip).
+RustError stage_upload(void* d_a, const void* a, size_t na, void* d_b, const void* b, size_t nb,
+    hipStream_t stream);
+
 static inline unsigned grid_for(uint64_t n, unsigned block, unsigned max_blocks = 65535u * 16u)
 {
     uint64_t g = (n + block - 1) / block;

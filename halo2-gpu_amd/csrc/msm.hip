@@ -30,6 +30,7 @@
 //  6. Horner over windows (c doublings per window), conversion to affine.
 // The result is written as (x, y, 1) or the identity (0, 1, 0) in Montgomery
 // form, the layout of the reference's `cukernel_to_affine` output.
+#include <atomic>
 #include <memory>
 #include <vector>
 
@@ -109,7 +110,8 @@ uint32_t log2_floor(uint64_t n)
 // plain MSMs of floor(log2 n) = i, entry 32 + i for precomputed tables of
 // floor(log2 table_n) = i. Set with h2amd_msm_set_widths before sizing,
 // launching and building tables.
-uint8_t g_width_table[64] = { 0 };
+// Atomic entries: a setter on one host thread may race planners on others.
+std::atomic<uint8_t> g_width_table[64] = {};
 
 // n (virtual) points of `bits`-bit signed-digit scalars
 uint32_t default_width(uint64_t n, bool shared, uint32_t bits)
@@ -126,7 +128,7 @@ uint32_t default_width(uint64_t n, bool shared, uint32_t bits)
 
 uint32_t plan_width(uint64_t n, bool shared)
 {
-    const uint32_t tuned = g_width_table[(log2_floor(n < 1 ? 1 : n) & 31) + (shared ? 32 : 0)];
+    const uint32_t tuned = g_width_table[(log2_floor(n < 1 ? 1 : n) & 31) + (shared ? 32 : 0)].load(std::memory_order_relaxed);
     if (tuned)
         return tuned;
     return default_width(n, shared, 255);
@@ -145,7 +147,7 @@ bool use_glv(uint64_t npoints) { return npoints < kGlvBelow; }
 
 uint32_t plain_msm_width(uint64_t npoints)
 {
-    const uint32_t tuned = g_width_table[log2_floor(npoints < 1 ? 1 : npoints) & 31];
+    const uint32_t tuned = g_width_table[log2_floor(npoints < 1 ? 1 : npoints) & 31].load(std::memory_order_relaxed);
     if (tuned)
         return tuned;
     return use_glv(npoints) ? default_width(2 * npoints, false, GLV_BITS) : default_width(npoints, false, 255);
@@ -1714,7 +1716,7 @@ RustError h2amd_msm_set_widths(const uint8_t* widths, uint32_t count)
         H2_CHECK_ARG(widths[i] == 0 || (widths[i] >= 4 && widths[i] <= (i < 32 ? 20 : 22)),
             "h2amd_msm_set_widths: width %u for entry %u outside [4, %u]", widths[i], i, i < 32 ? 20u : 22u);
     for (uint32_t i = 0; i < 64; ++i)
-        g_width_table[i] = i < count ? widths[i] : 0;
+        g_width_table[i].store(i < count ? widths[i] : 0, std::memory_order_relaxed);
     return ok();
 }
 

@@ -5,6 +5,8 @@
 // field29.h: elements stay in the boundary's 2^256 Montgomery scale and every
 // constant the pass multiplies by (twiddles, coset powers, zeta, the iFFT
 // scale) is held in the 2^261 scale.
+#include <atomic>
+#include <mutex>
 #include <vector>
 
 #include "field29.h"
@@ -81,7 +83,8 @@ static void host_to261(const Fr& x, uint32_t (&out)[9])
 
 // Tuned work-group tile sizes: log2 of the elements per work-group for
 // transforms of 2^log_n (0 = the default rule). Set with h2amd_ntt_set_tiles.
-static uint8_t g_tile_table[NTT_MAX_LOG + 1] = { 0 };
+// Atomic entries: a setter on one host thread may race planners on others.
+static std::atomic<uint8_t> g_tile_table[NTT_MAX_LOG + 1] = {};
 
 NttPlan ntt_plan(uint32_t log_n)
 {
@@ -107,8 +110,9 @@ NttPlan ntt_plan(uint32_t log_n)
             g = gmax;
         if (g < 0)
             g = 0;
-        if (log_n <= NTT_MAX_LOG && g_tile_table[log_n]) {
-            g = (int)g_tile_table[log_n] - (int)p.log_r[i];
+        const uint8_t tuned = log_n <= NTT_MAX_LOG ? g_tile_table[log_n].load(std::memory_order_relaxed) : 0;
+        if (tuned) {
+            g = (int)tuned - (int)p.log_r[i];
             if (g > gmax)
                 g = gmax;
             if (g > (int)log_n - (int)p.log_r[i])
@@ -417,7 +421,21 @@ __global__ void __launch_bounds__(1024) ntt_pass_batch_kernel(const BatchPass* _
     ntt_pass_tile(jb.in, jb.out, pp, blockIdx.x - jb.block0, lds);
 }
 
-static bool g_attr_set = false;
+// The dynamic-LDS limit of the pass kernels, raised once per process (std::call_once:
+// concurrent first calls from several host threads are safe).
+static hipError_t raise_lds_limits()
+{
+    static std::once_flag once;
+    static hipError_t status = hipSuccess;
+    std::call_once(once, [] {
+        status = hipFuncSetAttribute((const void*)ntt_pass_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+            36 << NTT_MAX_LOG_R);
+        if (status == hipSuccess)
+            status = hipFuncSetAttribute((const void*)ntt_pass_batch_kernel,
+                hipFuncAttributeMaxDynamicSharedMemorySize, 36 << NTT_MAX_LOG_R);
+    });
+    return status;
+}
 
 RustError ntt_run(const NttTables& t, uint32_t log_n, const NttJob* jobs, int count, uint64_t in_len,
     Fr* scratch, uint64_t scratch_slots, NttTwist twist, NttPost post, const Fr& scale, hipStream_t stream)
@@ -425,11 +443,7 @@ RustError ntt_run(const NttTables& t, uint32_t log_n, const NttJob* jobs, int co
     if (count <= 0)
         return ok();
     H2_CHECK_ARG(log_n >= 1 && log_n <= NTT_MAX_LOG, "NTT log_n=%u out of range [1, %u]", log_n, NTT_MAX_LOG);
-    if (!g_attr_set) {
-        H2_TRY(hipFuncSetAttribute((const void*)ntt_pass_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-            36 << NTT_MAX_LOG_R));
-        g_attr_set = true;
-    }
+    H2_TRY(raise_lds_limits());
     const NttPlan p = ntt_plan(log_n);
     const uint64_t N = 1ull << log_n;
     bool aliased = false;
@@ -516,12 +530,7 @@ RustError ntt_run_batch(const NttBatchJob* jobs, int count, NttTwist twist, NttP
 {
     if (count <= 0)
         return ok();
-    static bool attr = false;
-    if (!attr) {
-        H2_TRY(hipFuncSetAttribute((const void*)ntt_pass_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-            36 << NTT_MAX_LOG_R));
-        attr = true;
-    }
+    H2_TRY(raise_lds_limits());
     std::vector<BatchPass> host(3ull * count);
     uint32_t nlaunch[3] = { 0, 0, 0 }, grid[3] = { 0, 0, 0 }, rgmax[3] = { 0, 0, 0 };
     for (int i = 0; i < count; ++i) {
@@ -582,8 +591,12 @@ RustError ntt_run_batch(const NttBatchJob* jobs, int count, NttTwist twist, NttP
             host[(uint64_t)pass * count + nlaunch[pass]++] = bp;
         }
     }
-    H2_TRY(hipMemcpyAsync(d_pass, host.data(), sizeof(BatchPass) * host.size(), hipMemcpyHostToDevice, stream));
-    H2_TRY(hipStreamSynchronize(stream)); // the host table goes away on return
+    // pinned staging: the host table may go away on return without draining the stream
+    {
+        RustError e = stage_upload(d_pass, host.data(), sizeof(BatchPass) * host.size(), nullptr, nullptr, 0, stream);
+        if (e.code)
+            return e;
+    }
     const BatchPass* dp = (const BatchPass*)d_pass;
     for (int pass = 0; pass < 3; ++pass) {
         if (!nlaunch[pass])
@@ -608,7 +621,7 @@ RustError ntt_set_tiles(const uint8_t* tiles, uint32_t count)
         H2_CHECK_ARG(tiles[i] == 0 || (tiles[i] >= 6 && tiles[i] <= NTT_MAX_LOG_R),
             "h2amd_ntt_set_tiles: tile 2^%u for 2^%u outside [2^6, 2^%u]", tiles[i], i, NTT_MAX_LOG_R);
     for (uint32_t i = 0; i <= NTT_MAX_LOG; ++i)
-        g_tile_table[i] = i < count ? tiles[i] : 0;
+        g_tile_table[i].store(i < count ? tiles[i] : 0, std::memory_order_relaxed);
     return ok();
 }
 

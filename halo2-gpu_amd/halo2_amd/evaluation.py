@@ -18,6 +18,7 @@ of circuits through the same loop).
 from __future__ import annotations
 
 import ctypes
+import threading
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -54,6 +55,9 @@ class PermutationCommitted:
     product_polys: list
     columns: list
     sigma_polys: list = field(default_factory=list)
+
+
+_key_lock = threading.Lock()
 
 
 def _host_scalar(v: int) -> np.ndarray:
@@ -93,10 +97,15 @@ def evaluate_h_device(domain, gates: GraphEvaluator, fixed_polys, l0, l_last, l_
     for part in range(num_parts):
         key = key_parts.get(part) if key_parts is not None else None
         if key is None:
-            key = domain.coeff_to_extended_part_many_device(
-                list(fixed_polys) + [l0, l_last, l_active] + sigma_polys, cur)
-            if key_parts is not None:
-                key_parts[part] = key
+            with _key_lock:  # concurrent first proofs on one key: one builder per part
+                key = key_parts.get(part) if key_parts is not None else None
+                if key is None:
+                    key = domain.coeff_to_extended_part_many_device(
+                        list(fixed_polys) + [l0, l_last, l_active] + sigma_polys, cur)
+                    if key_parts is not None:
+                        # readers run on other streams: publish only once the parts are written
+                        torch.cuda.current_stream().synchronize()
+                        key_parts[part] = key
         fixed = key[:nf]
         l0_e, ll_e, la_e = key[nf:nf + 3]
         sig = key[nf + 3:]

@@ -8,6 +8,7 @@ Results are the reference's 96-byte projective-out layout ((n,) uint64[12]):
 from __future__ import annotations
 
 import ctypes
+import threading
 
 import numpy as np
 import torch
@@ -228,6 +229,7 @@ class ParamsKZG:
         self._tables: dict[str, torch.Tensor] = {}
         self._table_width: dict[str, int] = {}
         self._g2 = self._s_g2 = None
+        self._lock = threading.RLock()
 
     @classmethod
     def from_parts(cls, k: int, g_device: torch.Tensor, g_lagrange_device: torch.Tensor | None, g2, s_g2,
@@ -250,6 +252,7 @@ class ParamsKZG:
         self._tables = {}
         self._table_width = {}
         self._g2, self._s_g2 = g2, s_g2
+        self._lock = threading.RLock()
         return self
 
     def downsize(self, k: int) -> None:
@@ -296,9 +299,14 @@ class ParamsKZG:
         table width; a width changed since (h2amd_msm_set_widths) is refused."""
         width = int(_lib.load().h2amd_msm_table_width(self.n))
         if which not in self._tables:
-            bases = self.g_device if which == "g" else self.g_lagrange_device
-            self._tables[which] = msm_precompute(bases, self.n)
-            self._table_width[which] = width
+            with self._lock:  # concurrent first commits: one builder, readers see a finished table
+                if which not in self._tables:
+                    bases = self.g_device if which == "g" else self.g_lagrange_device
+                    table = msm_precompute(bases, self.n)
+                    # other threads launch on other streams: publish only once the table is written
+                    torch.cuda.current_stream().synchronize()
+                    self._table_width[which] = width
+                    self._tables[which] = table
         if self._table_width[which] != width:
             raise RuntimeError(f"the {which} table was built with window width {self._table_width[which]}, "
                                f"the library now uses {width}: rebuild the parameters")
@@ -307,21 +315,31 @@ class ParamsKZG:
     @property
     def g_lagrange_device(self) -> torch.Tensor:
         """commitment.rs:157-183 (scalars) and :223-241 (device mirror, built on first use)."""
-        if self._g_lagrange_device is None:
-            from . import polyops
-
-            n = self.n
-            roots = omega_powers(self.omega, self.k)
-            diff = dv.empty_u64(4 * n)
-            polyops.poly_fill_one_device(diff)  # diff = tau - w^i
-            polyops.poly_scale_device_with_d_s_minus_one(diff, _scalar_dev(self.tau - 1))
-            polyops.poly_sub_short_out_of_place_device(diff, diff, roots)
-            _lib.check(_lib.load()._halo2_batch_invert(dv.ptr(diff), 0, n, dv.stream()), "_halo2_batch_invert")
-            polyops.poly_elementwise_multiply_device(diff, diff, roots)
-            mult = (pow(self.tau, n, R_MOD) - 1) * pow(n, -1, R_MOD) % R_MOD
-            polyops.poly_scale_device_with_d_s_minus_one(diff, _scalar_dev(mult - 1))
-            self._g_lagrange_device = g1_mul_generator(diff, n)
+        if self._g_lagrange_device is not None:
+            return self._g_lagrange_device
+        with self._lock:
+            if self._g_lagrange_device is None:
+                self._g_lagrange_device = self._build_g_lagrange()
         return self._g_lagrange_device
+
+    def _build_g_lagrange(self) -> torch.Tensor:
+        """L_i(tau) G for every i, on the current stream, synchronised before it is
+        published (other threads read it from other streams)."""
+        from . import polyops
+
+        n = self.n
+        roots = omega_powers(self.omega, self.k)
+        diff = dv.empty_u64(4 * n)
+        polyops.poly_fill_one_device(diff)  # diff = tau - w^i
+        polyops.poly_scale_device_with_d_s_minus_one(diff, _scalar_dev(self.tau - 1))
+        polyops.poly_sub_short_out_of_place_device(diff, diff, roots)
+        _lib.check(_lib.load()._halo2_batch_invert(dv.ptr(diff), 0, n, dv.stream()), "_halo2_batch_invert")
+        polyops.poly_elementwise_multiply_device(diff, diff, roots)
+        mult = (pow(self.tau, n, R_MOD) - 1) * pow(n, -1, R_MOD) % R_MOD
+        polyops.poly_scale_device_with_d_s_minus_one(diff, _scalar_dev(mult - 1))
+        out = g1_mul_generator(diff, n)
+        torch.cuda.current_stream().synchronize()
+        return out
 
     def commit_device(self, d_poly: torch.Tensor) -> np.ndarray:
         """commitment.rs:560-577: MSM of device coefficients against the resident SRS

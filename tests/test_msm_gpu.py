@@ -135,9 +135,10 @@ def test_g1_mul_generator_matches_oracle(lib, co):
     assert (got == co.g1_mul_gen(s)).all()
 
 
-@pytest.mark.parametrize("k", [20, 22])
+@pytest.mark.parametrize("k", [20, 22, 24])
 def test_msm_closed_form_large(lib, co, k):
-    """sum c_i (s_i G) = (sum c_i s_i) G with device-generated bases and scalars."""
+    """sum c_i (s_i G) = (sum c_i s_i) G with device-generated bases and scalars. At 2^24
+    the plain path runs 20-bit windows (13 windows of 2^19 signed buckets) without GLV."""
     from halo2_amd import device as dv, msm
 
     n = 1 << k
@@ -198,6 +199,32 @@ def test_kzg_commit_srs_identity(lib, co):
     assert (params.commit(poly[:m]) == exp_m).all()
 
 
+@pytest.mark.parametrize("k", [22, 24])
+def test_kzg_commit_srs_identity_large(lib, co, k):
+    """The KZG commitment of the largest sizes, 2^22 and 2^24 points: f(tau) G through the
+    fixed-base table of the resident SRS (20-bit table windows: 13 rows of 2^k points) and
+    through the plain device MSM on the same bases, both against the closed form."""
+    import torch
+
+    from halo2_amd import device as dv, fields as F, msm
+
+    n = 1 << k
+    tau = (0x5EED0000 + k) * 0x9E3779B97F4A7C15 % F.R_MOD
+    print(f"kzg commit closed form k={k}", flush=True)
+    params = msm.ParamsKZG(k, tau)
+    dc = msm.random_fr(n, 101 + k)
+    pows = co.fr_powers(F.mont_limbs(tau), n)
+    hc = dv.to_host(dc)
+    exp = co.g1_mul(co.g1_generator(), co.fr_inner_product(hc, pows))
+    assert (params.commit_device(dc) == exp).all(), "fixed-base table"
+    assert (msm.multiexp_device_scalars_device_bases(dc, params.g_device, n) == exp).all(), "plain"
+    m = n - 12345  # a prefix of the table
+    exp_m = co.g1_mul(co.g1_generator(), co.fr_inner_product(hc[:m], pows[:m]))
+    assert (params.commit_device(dc[: 4 * m]) == exp_m).all(), "prefix"
+    del params
+    torch.cuda.empty_cache()
+
+
 def test_kzg_commit_lagrange(lib, co):
     """The Lagrange basis (commitment.rs:157-183): g_lagrange[i] = L_i(tau) G with
     L_i(tau) = (tau^n - 1)/n * w^i/(tau - w^i) on a sample of i, and commit_lagrange(v)
@@ -233,7 +260,7 @@ def _plan_width(n):
     return -(-255 // w0)
 
 
-@pytest.mark.parametrize("n", [1, 2, 17, 1000, 1 << 16, (1 << 20) + 3])
+@pytest.mark.parametrize("n", [1, 2, 17, 1000, 1 << 16, (1 << 20) + 3, 1 << 22])
 def test_msm_fixed_base_table(lib, co, n):
     """The fixed-base path equals the plain device MSM bit for bit: full length, prefixes
     of the table, identity bases and edge scalars; table rows are 2^(c w) P."""

@@ -356,3 +356,124 @@ def test_permute_expression_pair_seq(co, po):
         assert S[:U] == exp
     with pytest.raises(ValueError):
         co.permute_expression_pair(po.fr_array([1, 2]), po.fr_array([1, 1]), 2, 2)
+
+
+def _perm_instance(po, rng, n, ncols, blind, cycles_per, break_cell=None):
+    """A copy-constrained assignment over ncols columns of n rows: random cycles of 2..4
+    cells among the usable rows, every cell of a cycle equal to its first cell, sigma
+    values delta^col' omega^row' of the next cell of its cycle (permutation/keygen.rs).
+    break_cell = (col, row) changes one cell after the copies."""
+    R = po.R
+    u = n - blind - 1
+    w = po.omega(n.bit_length() - 1)
+    delta = pow(7, 1 << 28, R)
+    vals = [[rng.randrange(R) for _ in range(n)] for _ in range(ncols)]
+    nxt = {(c, r): (c, r) for c in range(ncols) for r in range(n)}
+    cells = [(c, r) for c in range(ncols) for r in range(u)]
+    rng.shuffle(cells)
+    i = 0
+    for _ in range(cycles_per):
+        ln = rng.randrange(2, 5)
+        cyc = cells[i:i + ln]
+        i += ln
+        for j, (c, r) in enumerate(cyc):
+            nxt[(c, r)] = cyc[(j + 1) % len(cyc)]
+            vals[c][r] = vals[cyc[0][0]][cyc[0][1]]
+    if break_cell is not None:
+        c, r = break_cell
+        vals[c][r] = (vals[c][r] + 1) % R
+    sigma = [[pow(delta, nxt[(c, r)][0], R) * pow(w, nxt[(c, r)][1], R) % R for r in range(n)]
+             for c in range(ncols)]
+    return vals, sigma, w, delta, u, nxt
+
+
+def _perm_products(po, vals, sigma, beta, gamma, w, delta, u, chunk_len, rng):
+    """The grand products of permutation/prover.rs:96-255 in big integers: set s covers
+    columns [s c, s c + c); z_0[0] = 1, z_s[0] = z_{s-1}[u]; z_s[i + 1] = z_s[i] *
+    prod_j (v_j + beta delta^j omega^i + gamma) / (v_j + beta sigma_j + gamma) for i < u;
+    rows after u random."""
+    R = po.R
+    n = len(vals[0])
+    zs, carry = [], 1
+    for s0 in range(0, len(vals), chunk_len):
+        z = [0] * n
+        z[0] = carry
+        for i in range(u):
+            num = den = 1
+            for j in range(s0, min(s0 + chunk_len, len(vals))):
+                num = num * (vals[j][i] + beta * pow(delta, j, R) * pow(w, i, R) + gamma) % R
+                den = den * (vals[j][i] + beta * sigma[j][i] + gamma) % R
+            z[i + 1] = z[i] * num * pow(den, -1, R) % R
+        for i in range(u + 1, n):
+            z[i] = rng.randrange(R)
+        carry = z[u]
+        zs.append(z)
+    return zs
+
+
+def test_permutation_quotient_multiset_vanishes_on_H(co, po):
+    """src/cpu/evaluator.rs:63-147 over several sets, evaluated on H itself (rot_scale 1,
+    l0 / l_last / l_active as indicator vectors): with grand products built from a valid
+    copy-constrained assignment (5 columns, chunk_len 2: three sets, the last partial)
+    every row of the quotient is zero -- the first-set start, the last-set closure, the
+    cross-set carries l0 (z_i - z_{i-1}(omega^last X)) and the per-set identities with
+    delta^j running across sets. A broken copy in the second set leaves the last set's
+    product open (z_last(u) != 1), and a wrong carry rotation, a delta restarted per set
+    or a dropped set term would each leave rows non-zero."""
+    rng = random.Random(2024)
+    R = po.R
+    n, ncols, blind, chunk = 32, 5, 3, 2
+    beta, gamma, y = (rng.randrange(R) for _ in range(3))
+
+    def quotient(vals, sigma, zs, w, delta, u, scale_delta_start=1):
+        l0 = [1] + [0] * (n - 1)
+        ll = [0] * n
+        ll[u] = 1
+        la = [1 if i < u else 0 for i in range(n)]
+        arr = lambda v: po.fr_array(v)  # noqa: E731
+        one = lambda v: po.fr_array([v])[0]  # noqa: E731
+        got = co.permutation_quotient(arr([0] * n), [arr(z) for z in zs], [arr(s) for s in sigma],
+                                      [arr(v) for v in vals], arr(l0), arr(ll), arr(la), one(beta), one(gamma),
+                                      one(y), one(delta), one(beta * scale_delta_start), one(1), one(w), chunk,
+                                      -(blind + 1), 1, n)
+        return po.fr_values(got)
+
+    vals, sigma, w, delta, u, nxt = _perm_instance(po, random.Random(77), n, ncols, blind, 12)
+    # at least one cycle must cross sets for the carries to matter
+    assert any(nxt[(c, r)][0] // chunk != c // chunk for c in range(ncols) for r in range(u))
+    zs = _perm_products(po, vals, sigma, beta, gamma, w, delta, u, chunk, rng)
+    assert len(zs) == 3 and zs[-1][u] == 1
+    assert quotient(vals, sigma, zs, w, delta, u) == [0] * n
+    # a broken copy in the second set: products rebuilt over the broken values do not close
+    cyc_cell = next((c, r) for c in (2, 3) for r in range(u) if nxt[(c, r)] != (c, r))
+    bvals, bsig, *_ = _perm_instance(po, random.Random(77), n, ncols, blind, 12, break_cell=cyc_cell)
+    assert bsig == sigma and sum(a != b for va, vb in zip(vals, bvals) for a, b in zip(va, vb)) == 1
+    bzs = _perm_products(po, bvals, bsig, beta, gamma, w, delta, u, chunk, random.Random(99))
+    assert bzs[-1][u] != 1
+    assert quotient(bvals, bsig, bzs, w, delta, u)[u] != 0
+    # the carry across sets must come from z_{i-1}(omega^last X): a product chain whose second
+    # set restarts at 1 breaks row 0
+    bad = [list(z) for z in zs]
+    ratio = pow(zs[1][0], -1, R)
+    bad[1] = [v * ratio % R for v in zs[1]]
+    bad[2] = [v * ratio % R for v in zs[2]]
+    got = quotient(vals, sigma, bad, w, delta, u)
+    assert got[0] != 0
+    # delta^j restarted per set (delta_start = beta delta^0 for every set) is not what the
+    # oracle computes: the same valid products leave active rows non-zero when built that way
+    zs_restart = []
+    carry = 1
+    for s0 in range(0, ncols, chunk):
+        z = [0] * n
+        z[0] = carry
+        for i in range(u):
+            num = den = 1
+            for jj, j in enumerate(range(s0, min(s0 + chunk, ncols))):
+                num = num * (vals[j][i] + beta * pow(delta, jj, R) * pow(w, i, R) + gamma) % R
+                den = den * (vals[j][i] + beta * sigma[j][i] + gamma) % R
+            z[i + 1] = z[i] * num * pow(den, -1, R) % R
+        for i in range(u + 1, n):
+            z[i] = zs[len(zs_restart)][i]
+        carry = z[u]
+        zs_restart.append(z)
+    assert any(v != 0 for v in quotient(vals, sigma, zs_restart, w, delta, u)[:u])
