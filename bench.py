@@ -226,8 +226,8 @@ def proof_measurements(msm, torch, k: int, reps: int) -> dict:
     from halo2_amd import prover
 
     circuit = prover.synthesize_standard(k, 5)
-    pk = prover.keygen(circuit)
     params = msm.ParamsKZG(k, 0x5EED)
+    pk = prover.keygen(circuit, params)
     prover.create_proof(params, pk, circuit, seed=99)  # warm-up: builds g_lagrange and the fixed-base tables
     lat, runs = [], []
     ta = time.perf_counter()
@@ -258,7 +258,6 @@ def serving_measurements(msm, torch, seconds: float, rate: float = 4.0, ks=(16, 
     fixed-base tables resident. Each k has two pre-generated witnesses used in turn
     (fresh blinders every proof). Reports proofs/s over the run and the latency from
     arrival to finished proof (queueing included)."""
-    import dataclasses
 
     import numpy as np
 
@@ -267,10 +266,10 @@ def serving_measurements(msm, torch, seconds: float, rate: float = 4.0, ks=(16, 
     setups = {}
     for k in ks:
         c = prover.synthesize_standard(k, seed + k)
-        pk = prover.keygen(c)
         params = msm.ParamsKZG(k, 0x5EED + k)
+        pk = prover.keygen(c, params)
         prover.create_proof(params, pk, c, seed=1)  # warm-up: g_lagrange, tables, key parts
-        witnesses = [c, dataclasses.replace(c, advice=prover.fresh_witness(c, seed + 100 + k))]
+        witnesses = [c, prover.fresh_witness(c, seed + 100 + k)]
         setups[k] = (params, pk, witnesses)
     rng = np.random.default_rng(seed)
     arrivals, t = [], 0.0

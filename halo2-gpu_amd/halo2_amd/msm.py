@@ -341,6 +341,17 @@ class ParamsKZG:
         torch.cuda.current_stream().synchronize()
         return out
 
+    def g0_out12(self) -> np.ndarray:
+        """g[0] (the generator G) in the 96-byte projective-out layout, read once."""
+        if getattr(self, "_g0", None) is None:
+            from .fields import P_MOD
+
+            out = np.zeros(12, dtype=np.uint64)
+            out[:8] = dv.to_host(self.g_device[:8], cols=8)[0]
+            out[8:12] = mont_limbs(1, P_MOD)
+            self._g0 = out
+        return self._g0
+
     def commit_device(self, d_poly: torch.Tensor) -> np.ndarray:
         """commitment.rs:560-577: MSM of device coefficients against the resident SRS
         (through its fixed-base table unless fixed_base is off)."""

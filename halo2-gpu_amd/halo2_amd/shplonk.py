@@ -103,13 +103,18 @@ def eval_host(coeffs: list[int], x: int) -> int:
     return acc
 
 
-def construct_intermediate_sets(queries: list[ProverQuery]) -> tuple[list[RotationSet], list[int]]:
-    """shplonk.rs:51-160: evaluate every query once (batched on the device), group the
-    points per commitment, then the commitments per point set. Commitment keys must be
-    hashable. Points keep their first appearance order (the reference iterates hash
-    sets; the results do not depend on the order)."""
-    evals = polyops.batch_eval_polynomial_d2h([q.poly for q in queries], [q.point for q in queries])
-    eval_of = {(q.commitment, q.point % R_MOD): value_of(e) for q, e in zip(queries, evals)}
+def construct_intermediate_sets(queries: list[ProverQuery],
+                                evals: list[int] | None = None) -> tuple[list[RotationSet], list[int]]:
+    """shplonk.rs:51-160: evaluate every query once (batched on the device, unless the
+    caller passes the values, query by query), group the points per commitment, then
+    the commitments per point set. Commitment keys must be hashable. Points keep their
+    first appearance order (the reference iterates hash sets; the results do not depend
+    on the order)."""
+    if evals is None:
+        evals = [value_of(e) for e in
+                 polyops.batch_eval_polynomial_d2h([q.poly for q in queries], [q.point for q in queries])]
+    assert len(evals) == len(queries)
+    eval_of = {(q.commitment, q.point % R_MOD): e % R_MOD for q, e in zip(queries, evals)}
     super_points: list[int] = []
     per_commitment: dict = {}  # key -> (poly, points), first-appearance order
     for q in queries:
@@ -142,14 +147,16 @@ def _zeros(n: int) -> torch.Tensor:
     return torch.zeros(4 * n, dtype=torch.int64, device=dv.device())
 
 
-def create_proof_device(params, queries: list[ProverQuery], y: int, v: int, u) -> dict:
+def create_proof_device(params, queries: list[ProverQuery], y: int, v: int, u,
+                        evals: list[int] | None = None) -> dict:
     """prover.rs:97-441 with the transcript's challenges given: returns the two commitments
     ("h", "h_final", 96-byte projective points as (12,) u64) and the device polynomials
     h_x (n coefficients) and h_x_final (n - 1 coefficients). `u` is the challenge or, as
     in the protocol (u is squeezed after h is written, prover.rs:388-392), a function
-    that takes h's commitment and returns u."""
+    that takes h's commitment and returns u. `evals`, when given, are the queries'
+    values (query by query), so they are not evaluated again."""
     n = params.n
-    rotation_sets, super_points = construct_intermediate_sets(queries)
+    rotation_sets, super_points = construct_intermediate_sets(queries, evals)
     for rs in rotation_sets:
         for c in rs.commitments:
             c.low_degree_equivalent = lagrange_interpolate(rs.points, c.evals)
