@@ -39,7 +39,8 @@ def parse():
     ap.add_argument("--log-n", type=int, default=24)
     ap.add_argument("--msm-logs", default="20,22,24")
     ap.add_argument("--msm-reps", type=int, default=3)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="threads of the CPU baseline: the job's CPU share (16 CPUs per GPU on the GPU box)")
     ap.add_argument("--no-msm", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rows", action="store_true", help="skip the per-row measurements (products, quotient, ...)")
@@ -250,55 +251,31 @@ def proof_measurements(msm, torch, k: int, reps: int) -> dict:
     return out
 
 
-def serving_measurements(msm, torch, seconds: float, rate: float = 4.0, ks=(16, 18, 20, 22),
-                         weights=(0.4, 0.3, 0.2, 0.1), seed: int = 55) -> dict:
-    """A proving service on one GPU: requests arrive as a Poisson process of `rate` per
-    second for `seconds`, each for the standard-gate circuit at k drawn with `weights`;
-    one server proves them first come first served with every k's proving key, SRS and
-    fixed-base tables resident. Each k has two pre-generated witnesses used in turn
-    (fresh blinders every proof). Reports proofs/s over the run and the latency from
-    arrival to finished proof (queueing included)."""
+def serving_measurements(torch, seconds: float, rates=(4.0, 20.0), ks=(16, 18, 20, 22),
+                         weights=(0.4, 0.3, 0.2, 0.1), seed: int = 55, slots: int = 4) -> dict:
+    """The proving service (halo2_amd.serving) on open-loop Poisson traces: requests for
+    the standard-gate circuit at k drawn with `weights`, each with a fresh witness, every
+    k's SRS, fixed-base tables and keys resident. Each trace is served twice: one proof
+    at a time (FIFO, the plain GPU path) and with continuous batching (`slots` proofs in
+    flight, admitted by their measured HBM footprint). The first rate is the config's
+    4/s (the device mostly idle); the second is near the one-at-a-time capacity (~25/s
+    for this mix), where queueing dominates FIFO latency."""
+    from halo2_amd import serving
 
-    import numpy as np
-
-    from halo2_amd import prover
-
-    setups = {}
-    for k in ks:
-        c = prover.synthesize_standard(k, seed + k)
-        params = msm.ParamsKZG(k, 0x5EED + k)
-        pk = prover.keygen(c, params)
-        prover.create_proof(params, pk, c, seed=1)  # warm-up: g_lagrange, tables, key parts
-        witnesses = [c, prover.fresh_witness(c, seed + 100 + k)]
-        setups[k] = (params, pk, witnesses)
-    rng = np.random.default_rng(seed)
-    arrivals, t = [], 0.0
-    while True:
-        t += float(rng.exponential(1.0 / rate))
-        if t > seconds:
-            break
-        arrivals.append((t, int(rng.choice(ks, p=weights))))
-    lat, per_k, ok, done = [], {k: [] for k in ks}, True, 0.0
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i, (ta, k) in enumerate(arrivals):
-        wait = ta - (time.perf_counter() - t0)
-        if wait > 0:
-            time.sleep(wait)
-        params, pk, wits = setups[k]
-        r = prover.create_proof(params, pk, wits[i % 2], seed=1000 + i)
-        torch.cuda.current_stream().synchronize()
-        done = time.perf_counter() - t0
-        lat.append((done - ta) * 1e3)
-        per_k[k].append((done - ta) * 1e3)
-        ok &= all(r["checks"].values())
-    return {"rate_per_s": rate, "seconds": seconds, "ks": list(ks), "weights": list(weights),
-            "requests": len(arrivals), "proofs_per_s": round(len(arrivals) / max(done, seconds), 3),
-            "p50_ms": round(float(np.percentile(lat, 50)), 2) if lat else None,
-            "p99_ms": round(float(np.percentile(lat, 99)), 2) if lat else None,
-            "by_k": {str(k): {"n": len(v), "p50_ms": round(float(np.percentile(v, 50)), 2) if v else None}
-                     for k, v in per_k.items()},
-            "all_checks_hold": bool(ok)}
+    setups = {k: serving.prepare_setup(k, seed + k, 0x5EED + k) for k in ks}
+    out = {"seconds": seconds, "ks": list(ks), "weights": list(weights), "slots": slots,
+           "footprint_mib": {str(k): round(s.footprint / 2**20, 1) for k, s in setups.items()}, "runs": {}}
+    for rate in rates:
+        trace = serving.poisson_trace(seconds, rate, ks, weights, seed + int(rate * 10))
+        runs = {}
+        for name, nslots in (("fifo", 1), ("continuous", slots)):
+            r = serving.run_trace(setups, trace, nslots, seed=seed * 1000)
+            r.pop("results")
+            runs[name] = r
+        out["runs"][f"rate_{rate:g}"] = runs
+    del setups
+    torch.cuda.empty_cache()
+    return out
 
 
 def proofs_in_flight(prover, params, pk, circuit, torch, streams: int, per_stream: int) -> dict:
@@ -337,6 +314,66 @@ def proofs_in_flight(prover, params, pk, circuit, torch, streams: int, per_strea
     return {"streams": streams, "proofs": len(res), "proofs_per_s": round(len(res) / wall, 3),
             "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
             "all_checks_hold": all(v[1] for v in res)}
+
+
+def _median_time(fn, reps: int) -> float:
+    fn()  # warm-up
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t)
+    return float(sorted(ts)[len(ts) // 2])
+
+
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_measurements(msm, dv, F, k: int, max_threads: int):
+    """The C restatement of the reference CPU path (oracle/, the checker's code, timed
+    here as the baseline) on the same seeded inputs: the headline 2^k coset NTT
+    (distribute c^i + radix-2 FFT) on the job's CPU share, and beside it the MSM
+    (multiexp_serial / best_multiexp_cpu, src/cpu/arithmetic.rs:14-156) at 2^16 and
+    2^20, the 2^16 NTT, and single-thread figures. Each is the median of 5 after a
+    warm-up (3 for the single-thread ones)."""
+    from oracle import coracle as co
+
+    avail = len(os.sched_getaffinity(0))
+    threads = max(1, min(max_threads, avail))
+    w = F.mont_limbs(F.omega(k))
+    c = F.mont_limbs(F.ZETA)
+    a = dv.to_host(msm.random_fr(1 << k, seed=2024))
+
+    def coset_ntt(arr, kk, th):
+        t = co.distribute_powers_inplace(arr.copy(), c, th)
+        co.fft(t, kk, F.mont_limbs(F.omega(kk)), th)
+
+    dt = _median_time(lambda: coset_ntt(a, k, threads), 5)
+    cpu = {"value": (1 << k) / dt, "unit": "elems/s", "cores": threads, "kind": "port",
+           "sample": f"median of 5 (after a warm-up) full coset NTTs of 2^{k} (distribute c^i + radix-2 DIT "
+                     f"FFT; C restatement of the reference CPU path) on {threads} threads"}
+    del a
+    host = {"cpu_model": _cpu_model(), "nproc": os.cpu_count(), "affinity_cpus": avail, "threads_used": threads,
+            "note": "threads = the job's CPU share (16 on the one-GPU box), not every core the host reports"}
+    a16 = co.chacha_fr(31, 0, 1 << 16)
+    a20 = co.chacha_fr(32, 0, 1 << 20)
+    host["ntt_coset_2^16_elems_per_s"] = (1 << 16) / _median_time(lambda: coset_ntt(a16, 16, threads), 5)
+    host["ntt_coset_2^20_1thread_elems_per_s"] = (1 << 20) / _median_time(lambda: coset_ntt(a20, 20, 1), 3)
+    for lk in (16, 20):
+        m = 1 << lk
+        s_ = co.chacha_fr(40 + lk, 0, m)
+        pts = co.g1_mul_gen(co.chacha_fr(50 + lk, 0, m), threads)
+        host[f"msm_2^{lk}_elems_per_s"] = m / _median_time(lambda: co.msm(s_, pts, threads), 5)
+        if lk == 16:
+            host["msm_2^16_1thread_elems_per_s"] = m / _median_time(lambda: co.msm(s_, pts, 1), 3)
+    return cpu, host
 
 
 def main() -> int:
@@ -397,9 +434,12 @@ def main() -> int:
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = n / (ms_per_step * 1e-3)
     avg_pass_ms = pass_ms / max(1, launches)
-    pass_bytes = 64 * n  # one 32-byte read and one 32-byte write per element per pass
-    achieved = pass_bytes / (avg_pass_ms * 1e-3) / 1e9
     passes_per_step = launches / max(1, args.steps)
+    # algorithmic bytes: 64 N per transform (one compulsory 32-byte read and write per
+    # element); a launch (one of the transform's passes) is credited its share of them
+    transform_bytes = 64 * n
+    pass_bytes = transform_bytes / passes_per_step
+    achieved = pass_bytes / (avg_pass_ms * 1e-3) / 1e9
 
     # VALU view: field multiplications of the transform vs the measured multiply-add peak
     probe = dv.empty_bytes(2048 * 256 * 8)
@@ -413,7 +453,10 @@ def main() -> int:
         "ntt_2^%d_ms" % k: round(ms_per_step, 4),
         "ntt_pass_avg_ms": round(avg_pass_ms, 4),
         "ntt_pass_launches_per_step": passes_per_step,
-        "ntt_whole_step_hbm_frac": round(pass_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "ntt_whole_step_hbm_frac": round(transform_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        # each pass streams the whole array once: 64 N bytes moved per launch
+        "ntt_pass_bytes_moved_per_launch": 64 * n,
+        "ntt_pass_moved_gbs": round(64 * n / (avg_pass_ms * 1e-3) / 1e9, 1),
         "mad_u64_u32_peak_per_s": peak.value,
         "ntt_valu_frac_of_mad_peak": round(valu_frac, 4),
     }
@@ -498,7 +541,7 @@ def main() -> int:
 
     # ---------------- proving service: Poisson arrivals over k in {16..22}
     if not args.no_proof and args.serve_seconds > 0:
-        extra["serving"] = serving_measurements(msm, torch, args.serve_seconds)
+        extra["serving"] = serving_measurements(torch, args.serve_seconds)
 
     # ------------------ tuner: MSM width / NTT tile per size vs the default rules
     if not args.no_autotune:
@@ -516,21 +559,7 @@ def main() -> int:
     # ------------------------------------------------------ CPU baseline
     cpu = None
     if not args.no_cpu:
-        from oracle import coracle as co
-
-        threads = max(1, min(args.cpu_threads, len(os.sched_getaffinity(0))))
-        a = dv.to_host(msm.random_fr(n, seed=2024))
-        w = F.mont_limbs(F.omega(k))
-        c = F.mont_limbs(F.ZETA)
-        reps = 2
-        ta = time.perf_counter()
-        for _ in range(reps):
-            t = co.distribute_powers_inplace(a.copy(), c, threads)
-            co.fft(t, k, w, threads)
-        dt = (time.perf_counter() - ta) / reps
-        cpu = {"value": n / dt, "unit": "elems/s", "cores": threads, "kind": "port",
-               "sample": f"{reps} full coset NTTs of 2^{k} (distribute c^i + radix-2 DIT FFT, "
-                         f"C restatement of the reference CPU path)"}
+        cpu, extra["cpu_baselines"] = cpu_measurements(msm, dv, F, k, args.cpu_threads)
 
     line = {
         "metric": "MSM/NTT elems/s at 2^16–2^24; proofs/s + p99 latency at k=20 vs plain GPU",
