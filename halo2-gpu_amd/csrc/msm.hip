@@ -31,6 +31,7 @@
 // The result is written as (x, y, 1) or the identity (0, 1, 0) in Montgomery
 // form, the layout of the reference's `cukernel_to_affine` output.
 #include <atomic>
+#include <mutex>
 #include <memory>
 #include <vector>
 
@@ -1460,6 +1461,69 @@ RustError _halo2_multiexp_device_scalars_device_bases(const void* d_scalar, cons
 {
     return msm_entry("_halo2_multiexp_device_scalars_device_bases", d_scalar, true, d_bases, true, h_output->ptr,
         length, scratch, scratch_bytes, (hipStream_t)stream);
+}
+
+// ------------------------------- scratch-free entry point (no reference counterpart)
+//
+// For callers that cannot allocate device memory (code under graph capture, tight
+// serving loops): the MSM's scratch lives in a file-scope device array, one per GPU,
+// held until the process ends. Every call on a GPU shares that GPU's array, so calls
+// that overlap on one GPU race; the pipeline clears what must start at zero itself.
+
+}  // extern "C"
+
+namespace {
+
+constexpr uint64_t kNoAllocMaxLength = 1ull << 16;
+constexpr uint64_t kNoAllocScratchBytes = 64ull << 20; // >= msm_workspace_bytes(2^16) at default widths
+
+__device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kNoAllocMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per device
+// (the array's name is a host-side stand-in and must never reach a kernel).
+RustError noalloc_scratch(const char* who, void** out)
+{
+    int dev = 0;
+    H2_TRY(hipGetDevice(&dev));
+    H2_CHECK_ARG(dev >= 0 && dev < kNoAllocMaxDevices, "%s: device %d beyond the %d supported", who, dev,
+        kNoAllocMaxDevices);
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* ptr = nullptr;
+        H2_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        H2_CHECK_ARG(ptr != nullptr, "%s: no device address for the built-in scratch", who);
+        g_noalloc_ptr[dev] = ptr;
+    }
+    *out = g_noalloc_ptr[dev];
+    return ok();
+}
+
+} // namespace
+
+extern "C" {
+
+uint64_t h2amd_multiexp_noalloc_max_length(void) { return kNoAllocMaxLength; }
+
+RustError h2amd_multiexp_device_scalars_device_bases_noalloc(const void* d_scalar, const void* d_bases,
+    const FFITraitObject* h_output, size_t length, void* stream)
+{
+    const char* who = "h2amd_multiexp_device_scalars_device_bases_noalloc";
+    H2_CHECK_ARG(length <= kNoAllocMaxLength, "%s: length %lu exceeds the built-in scratch's maximum of %lu points",
+        who, (unsigned long)length, (unsigned long)kNoAllocMaxLength);
+    const uint64_t need = msm_workspace_bytes(length);
+    H2_CHECK_ARG(need <= kNoAllocScratchBytes,
+        "%s: length %lu needs %lu bytes of scratch at the current window width, the built-in scratch holds %lu", who,
+        (unsigned long)length, (unsigned long)need, (unsigned long)kNoAllocScratchBytes);
+    void* ws = nullptr;
+    RustError e = noalloc_scratch(who, &ws);
+    if (e.code)
+        return e;
+    return msm_entry(who, d_scalar, true, d_bases, true, h_output->ptr, length, ws, kNoAllocScratchBytes,
+        (hipStream_t)stream);
 }
 
 // ----------------------------------------- fixed bases (no reference counterpart)

@@ -64,6 +64,8 @@ SIGNATURES = {
     "_halo2_multiexp_device_bases": (RustError, [OBJP, P, OBJP, SZ, P, U64, P]),
     "_halo2_multiexp_device_scalars_device_bases_workspace_size": (U64, [U64]),
     "_halo2_multiexp_device_scalars_device_bases": (RustError, [P, P, OBJP, SZ, P, U64, P]),
+    "h2amd_multiexp_noalloc_max_length": (U64, []),
+    "h2amd_multiexp_device_scalars_device_bases_noalloc": (RustError, [P, P, OBJP, SZ, P]),
     "_halo2_fft_normal_workspace_size": (U64, [U32, U32, U32]),
     "_halo2_fft_normal": (RustError, [U32, U32, P, P, P, P, P, U64, P]),
     "_halo2_fft_normal_check_memory": (BOOL, [U32, P, U32, U32]),

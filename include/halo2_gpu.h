@@ -79,6 +79,17 @@ uint64_t _halo2_multiexp_device_scalars_device_bases_workspace_size(uint64_t len
 RustError _halo2_multiexp_device_scalars_device_bases(const void* d_scalar, const void* d_bases,
     const FFITraitObject* h_output, size_t length, void* scratch, uint64_t scratch_bytes, void* stream);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): _halo2_multiexp_device_scalars_device_bases without the workspace
+ * argument. Its scratch is a 64 MiB device array built into the library, one per
+ * GPU, held until the process ends; lengths above h2amd_multiexp_noalloc_max_length()
+ * (2^16 points) are rejected with an error. Calls that overlap on one GPU share that
+ * array and race. Allocates nothing: no hipMalloc/hipFree. */
+uint64_t h2amd_multiexp_noalloc_max_length(void);
+RustError h2amd_multiexp_device_scalars_device_bases_noalloc(const void* d_scalar, const void* d_bases,
+    const FFITraitObject* h_output, size_t length, void* stream);
+
 /* ================================================================ NTT */
 
 /* culib.rs:492 ; cuda/src/ntt.cu:383-393 */
