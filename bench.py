@@ -53,9 +53,18 @@ def parse():
     ap.add_argument("--no-proof", action="store_true", help="skip the end-to-end proof measurement")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the Poisson-arrival serving run (0 skips it)")
+    ap.add_argument("--serve-rates", default="4,28", help="arrival rates (per second) of the serving runs")
+    ap.add_argument("--serve-modes", default="fifo,continuous", help="serving modes to run")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the serving measurement (e.g. --serve-seconds 600 --serve-rates 4)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one plain device copy of this size first (FETCH_SIZE/WRITE_SIZE calibration)")
     return ap.parse_args()
+
+
+def progress(msg: str) -> None:
+    """One progress line on stderr (stdout carries only the result line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
 def _timed(fn, reps, torch):
@@ -251,15 +260,17 @@ def proof_measurements(msm, torch, k: int, reps: int) -> dict:
     return out
 
 
-def serving_measurements(torch, seconds: float, rates=(4.0, 20.0), ks=(16, 18, 20, 22),
-                         weights=(0.4, 0.3, 0.2, 0.1), seed: int = 55, slots: int = 4) -> dict:
+def serving_measurements(torch, seconds: float, rates=(4.0, 28.0), ks=(16, 18, 20, 22),
+                         weights=(0.4, 0.3, 0.2, 0.1), seed: int = 55, slots: int = 4,
+                         modes=("fifo", "continuous")) -> dict:
     """The proving service (halo2_amd.serving) on open-loop Poisson traces: requests for
     the standard-gate circuit at k drawn with `weights`, each with a fresh witness, every
     k's SRS, fixed-base tables and keys resident. Each trace is served twice: one proof
     at a time (FIFO, the plain GPU path) and with continuous batching (`slots` proofs in
     flight, admitted by their measured HBM footprint). The first rate is the config's
-    4/s (the device mostly idle); the second is near the one-at-a-time capacity (~25/s
-    for this mix), where queueing dominates FIFO latency."""
+    4/s (the device mostly idle); the second is just above the one-at-a-time capacity
+    (~25/s for this mix: a mean service time of ~40 ms), so FIFO falls behind and its
+    queue grows while the batched service, whose capacity is higher, keeps up."""
     from halo2_amd import serving
 
     setups = {k: serving.prepare_setup(k, seed + k, 0x5EED + k) for k in ks}
@@ -269,7 +280,10 @@ def serving_measurements(torch, seconds: float, rates=(4.0, 20.0), ks=(16, 18, 2
         trace = serving.poisson_trace(seconds, rate, ks, weights, seed + int(rate * 10))
         runs = {}
         for name, nslots in (("fifo", 1), ("continuous", slots)):
-            r = serving.run_trace(setups, trace, nslots, seed=seed * 1000)
+            if name not in modes:
+                continue
+            progress(f"serving rate {rate:g}/s {name}: {len(trace)} requests")
+            r = serving.run_trace(setups, trace, nslots, seed=seed * 1000, progress=progress)
             r.pop("results")
             runs[name] = r
         out["runs"][f"rate_{rate:g}"] = runs
@@ -392,6 +406,12 @@ def main() -> int:
     lib = _lib.load()
     torch.cuda.set_device(0)
     stream = dv.stream()
+    if args.serve_only:  # a long serving run on its own (its result line is not the benchmark line)
+        rates = tuple(float(x) for x in args.serve_rates.split(",") if x)
+        modes = tuple(x for x in args.serve_modes.split(",") if x)
+        print(json.dumps({"serving": serving_measurements(torch, args.serve_seconds, rates=rates, modes=modes)}),
+              flush=True)
+        return 0
     k = args.log_n
     n = 1 << k
 
@@ -405,6 +425,7 @@ def main() -> int:
         del src, dst
 
     # ---------------------------------------------------------------- NTT
+    progress(f"coset NTT 2^{k}")
     d_in = msm.random_fr(n, seed=2024)
     d_out = dv.empty_u64(4 * n)
     ws = int(lib._halo2_fft_normal_to_device_workspace_size(6, k, k))
@@ -485,6 +506,7 @@ def main() -> int:
         extra["ntt_2^%d_elems_per_s" % kk] = nn / ((time.perf_counter() - ta) / reps)
 
     # ---------------------------------------------------------------- MSM
+    progress("MSM")
     if not args.no_msm:
         logs = [int(x) for x in args.msm_logs.split(",") if x]
         kmax = max(logs)
@@ -528,22 +550,28 @@ def main() -> int:
         del srs
 
     # ----------------------------------------- the other hot-path rows
+    progress("rows")
     if not args.no_rows:
         extra["rows"] = row_measurements(lib, dv, F, msm, torch, stream)
 
     # ------------------------------- uneven batch (32 polys, k in 14..22)
+    progress("uneven batch")
     if not args.no_batch:
         extra["batch_uneven_32"] = batch_measurements(msm, torch)
 
     # ------------------------- end-to-end proof of the standard-gate circuit
+    progress("proofs")
     if not args.no_proof:
         extra[f"proof_k{args.proof_k}"] = proof_measurements(msm, torch, args.proof_k, args.proof_reps)
 
     # ---------------- proving service: Poisson arrivals over k in {16..22}
+    progress("serving")
     if not args.no_proof and args.serve_seconds > 0:
-        extra["serving"] = serving_measurements(torch, args.serve_seconds)
+        extra["serving"] = serving_measurements(torch, args.serve_seconds,
+                                                rates=tuple(float(x) for x in args.serve_rates.split(",") if x))
 
     # ------------------ tuner: MSM width / NTT tile per size vs the default rules
+    progress("tuner")
     if not args.no_autotune:
         from halo2_amd import autotune
 
@@ -557,6 +585,7 @@ def main() -> int:
         }
 
     # ------------------------------------------------------ CPU baseline
+    progress("CPU baseline")
     cpu = None
     if not args.no_cpu:
         cpu, extra["cpu_baselines"] = cpu_measurements(msm, dv, F, k, args.cpu_threads)

@@ -310,10 +310,24 @@ __device__ __forceinline__ void ntt_pass_tile(
         }
         s = 1;
     }
+    // A stage pair of half-width h touches, per wave, the 256 slots
+    // [256 w', 256 w' + 256) that the same wave wrote in the previous pair whenever
+    // its blocks (4h slots) fit in 256 slots or a column has at most 256 slots
+    // (q -> slot ranges are a fixed function of q for every such pair): those
+    // pairs need only wave-level ordering of the LDS accesses (LDS serves one
+    // wave's instructions in order), not a work-group barrier. The first pair
+    // after the load or the radix-2 stage reads other waves' slots.
     const uint32_t quarterR = R >> 2;
+    const uint32_t s_first = s;
     for (; s + 1 < log_r; s += 2) {
-        __syncthreads();
         const uint32_t h = 1u << s;
+        if (s == s_first || ((4u << s) > 256u && R > 256u)) {
+            __syncthreads();
+        } else {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
         for (uint32_t q = threadIdx.x; q < (RG >> 2); q += T) {
             const uint32_t g = q >> (log_r - 2);
             const uint32_t qq = q & (quarterR - 1);

@@ -171,22 +171,31 @@ def poisson_trace(seconds: float, rate: float, ks, weights, seed: int) -> list:
         out.append((t, int(rng.choice(ks, p=weights))))
 
 
-def run_trace(setups: dict, trace: list, slots: int, seed: int = 0, hbm_budget: int | None = None) -> dict:
+def run_trace(setups: dict, trace: list, slots: int, seed: int = 0, hbm_budget: int | None = None,
+              progress=None, progress_every_s: float = 30.0) -> dict:
     """Open-loop run: every request is submitted at its arrival time whatever the
     service's state; each gets a fresh witness (seed + index) and blinders. Returns
     throughput, latency percentiles (arrival to finished proof), per-k latencies and the
-    peak number of proofs in flight."""
+    peak number of proofs in flight. `progress(message)`, when given, is called about
+    every `progress_every_s` seconds of the run."""
     svc = ProvingService(setups, slots=slots, hbm_budget=hbm_budget)
     reqs = []
     torch.cuda.synchronize()
     svc.t0 = time.perf_counter()
+    last = 0.0
     for i, (ta, k) in enumerate(trace):
         wait = ta - svc.now()
         if wait > 0:
             time.sleep(wait)
         reqs.append(svc.submit(Request(k, witness_seed=seed + 7 * i + 1, blind_seed=seed + 7 * i + 3, arrival=ta)))
+        if progress is not None and svc.now() - last >= progress_every_s:
+            last = svc.now()
+            done = sum(r.event.is_set() for r in reqs)
+            progress(f"t={last:.0f}s submitted {len(reqs)} done {done} in flight {svc.in_flight}")
     for r in reqs:
-        r.event.wait()
+        while not r.event.wait(progress_every_s if progress is not None else None):
+            done = sum(x.event.is_set() for x in reqs)
+            progress(f"t={svc.now():.0f}s draining: done {done} of {len(reqs)}")
     svc.close()
     errors = [r.error for r in reqs if r.error is not None]
     if errors:
