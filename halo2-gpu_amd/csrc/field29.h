@@ -135,6 +135,19 @@ struct F29 {
         q[1] = make_uint4(v[4], v[5], v[6], v[7]);
     }
 
+    // 32 a as an integer, normalised limbs: a normalised word x * 2^256 (value < 2^256)
+    // becomes x * 2^261, the same element in the 2^261 scale (value < 2^261), so that
+    // mul(shl5(a), b) of two stored words is the stored word of their product
+    __device__ __forceinline__ static F29 shl5(const F29& a)
+    {
+        F29 r;
+        r.l[0] = (a.l[0] << 5) & MASK;
+#pragma unroll
+        for (int i = 1; i < 9; ++i)
+            r.l[i] = ((a.l[i] << 5) & MASK) | (a.l[i - 1] >> 24);
+        return r;
+    }
+
     // carry limbs 0..7 into their successors (value unchanged)
     __device__ __forceinline__ static F29 normalize(const F29& a)
     {

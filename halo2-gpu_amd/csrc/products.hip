@@ -5,14 +5,17 @@
 //
 // Batch inversion: Montgomery's trick at two levels. Each thread folds 8
 // elements (zeros are skipped and stay zero, ff::BatchInvert semantics), the
-// 256 thread products of a work-group are combined with LDS prefix/suffix
-// scans, and one field inversion per 2048 elements finishes the work-group.
+// 256 thread products of a work-group are inverted through a product tree in
+// LDS, and one field inversion per 2048 elements finishes the work-group.
 //
 // Grand product: in-place multiplicative scan. A level scans chunks of 2048
-// elements locally; the last element of every chunk (the chunk product) forms
-// a strided view that is scanned recursively in place, and a fix-up pass
-// multiplies each chunk by the scanned product of the chunks before it. No
-// scratch memory is needed, matching the scratch-free device entry point.
+// elements locally (thread runs, then the exclusive prefixes of the thread
+// totals from the LDS product tree); the last element of every chunk (the chunk
+// product) forms a strided view that is scanned recursively in place, and a
+// fix-up pass multiplies each chunk by the scanned product of the chunks before
+// it. No scratch memory is needed, matching the scratch-free device entry point.
+//
+// Everything here runs on the 29-bit limb form of field29.h.
 #include "common.h"
 #include "field.h"
 #include "field29.h"
@@ -25,29 +28,11 @@ constexpr unsigned T = 256; // threads per work-group
 constexpr unsigned E = 8; // elements per thread
 constexpr unsigned C = T * E; // elements per work-group
 
-// inclusive Hillis-Steele product scan over the T values in s (in place)
-template <class F>
-__device__ void block_scan_inclusive(F* s, unsigned tid, bool reverse)
-{
-    for (unsigned d = 1; d < T; d <<= 1) {
-        __syncthreads();
-        F v;
-        bool take;
-        if (!reverse) {
-            take = tid >= d;
-            if (take)
-                v = s[tid - d];
-        } else {
-            take = tid + d < T;
-            if (take)
-                v = s[tid + d];
-        }
-        __syncthreads();
-        if (take)
-            s[tid] = F::mul(v, s[tid]);
-    }
-    __syncthreads();
-}
+using R29 = Fr29;
+
+__device__ __forceinline__ R29 ld29(const Fr* p) { return R29::load32(p); }
+// any limbs below 2^32 and value below 2^261 -> the canonical stored word
+__device__ __forceinline__ void st29(const R29& x, Fr* p) { R29::reduce_full(R29::normalize(x)).store32(p); }
 
 // Inverses of the T thread products through a product tree in LDS (heap layout:
 // node 1 is the root, node i has children 2i and 2i + 1, leaf t is node T + t):
@@ -513,35 +498,73 @@ RustError batch_invert(F* data, uint64_t n, hipStream_t stream, void* ws = nullp
 // ------------------------------------------------------------------- scan
 // Strided view A[t] = base[t * s], t < m.
 
+// The scan runs on the 29-bit form (field29.h). Running products stay in the
+// 2^261 scale (every loaded word x * 2^256 enters as shl5, x * 2^261), the exclusive
+// prefix of the thread totals comes from the product tree in LDS (up-sweep, then the
+// prefixes pushed down: a left child inherits its parent's prefix, a right child the
+// parent's prefix times its left sibling), and the output product v * p takes p in
+// the 2^256 scale (the tree prefix times the caller's prefix word, or times 2^256
+// mod r), so it lands back in the stored scale.
+
+// 2^256 mod r in 29-bit limbs: mul(v * 2^261, ONE256) = v * 2^256
+__device__ __forceinline__ R29 one256()
+{
+    constexpr uint32_t k[9] = { 0x0ffffffbu, 0x04b1a0e2u, 0x18334a6bu, 0x18ed2b3eu, 0x1462e36fu,
+        0x11b7bc3cu, 0x1cbd99bau, 0x183340fbu, 0x000e0a77u };
+    R29 r;
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+        r.l[i] = k[i];
+    return r;
+}
+
+// The caller has replaced node[1] by the identity of the prefixes (and
+// synchronised); returns the product of the leaves before leaf tid.
+template <class F>
+__device__ __forceinline__ F tree_down_prefix(F* node, unsigned tid)
+{
+    for (unsigned half = 1; half < T; half <<= 1) {
+        if (tid < half) {
+            const unsigned i = half + tid;
+            const F p = node[i];
+            const F left = node[2 * i];
+            node[2 * i] = p;
+            node[2 * i + 1] = F::mul(p, left);
+        }
+        __syncthreads();
+    }
+    return node[T + tid];
+}
+
 // Local inclusive scan of each chunk of C view elements. With `prefix` non-null
 // (single-chunk top level) every element is also multiplied by *prefix.
 __global__ void __launch_bounds__(T) scan_local_kernel(Fr* __restrict__ base, uint64_t s, uint64_t m,
     const Fr* __restrict__ prefix)
 {
-    __shared__ Fr tot[T];
+    __shared__ R29 node[2 * T];
     const unsigned tid = threadIdx.x;
     const uint64_t c0 = (uint64_t)blockIdx.x * C + (uint64_t)tid * E;
-    Fr v[E];
-    Fr acc = Fr::one();
-#pragma unroll
-    for (unsigned k = 0; k < E; ++k) {
-        const uint64_t t = c0 + k;
-        if (t < m) {
-            acc = Fr::mul(acc, Fr::load(base + t * s));
-        }
-        v[k] = acc;
-    }
-    tot[tid] = acc;
-    block_scan_inclusive(tot, tid, false);
-    Fr p = tid > 0 ? tot[tid - 1] : Fr::one();
-    if (prefix)
-        p = Fr::mul(p, Fr::load(prefix));
-    const bool apply = tid > 0 || prefix != nullptr;
+    R29 v[E];
+    R29 acc = R29::one(); // 2^261 scale
 #pragma unroll
     for (unsigned k = 0; k < E; ++k) {
         const uint64_t t = c0 + k;
         if (t < m)
-            (apply ? Fr::mul(v[k], p) : v[k]).store(base + t * s);
+            acc = R29::mul(acc, R29::shl5(ld29(base + t * s)));
+        v[k] = acc;
+    }
+    node[T + tid] = acc;
+    tree_up(node, tid);
+    if (tid == 0)
+        node[1] = R29::one();
+    __syncthreads();
+    const R29 pre = tree_down_prefix(node, tid); // 2^261 scale
+    const R29 p = R29::mul(pre, prefix ? ld29(prefix) : one256()); // 2^256 scale
+#pragma unroll
+    for (unsigned k = 0; k < E; ++k) {
+        const uint64_t t = c0 + k;
+        if (t < m)
+            st29(R29::mul(v[k], p), base + t * s);
     }
 }
 
@@ -552,7 +575,7 @@ __global__ void __launch_bounds__(T) scan_fixup_kernel(Fr* __restrict__ base, ui
 {
     const uint64_t u = blockIdx.x;
     const uint64_t nchunks = (m + C - 1) / C;
-    const Fr f = u == 0 ? Fr::load(prefix) : Fr::load(base + (u * C - 1) * s);
+    const R29 f = ld29(u == 0 ? prefix : base + (u * C - 1) * s);
     const uint64_t first = u * C;
     uint64_t last = first + C; // exclusive
     if (u + 1 < nchunks)
@@ -560,7 +583,7 @@ __global__ void __launch_bounds__(T) scan_fixup_kernel(Fr* __restrict__ base, ui
     if (last > m)
         last = m;
     for (uint64_t t = first + threadIdx.x; t < last; t += T)
-        Fr::mul(Fr::load(base + t * s), f).store(base + t * s);
+        st29(R29::mul(R29::shl5(ld29(base + t * s)), f), base + t * s);
 }
 
 RustError scan_view(Fr* base, uint64_t s, uint64_t m, const Fr* prefix, hipStream_t stream)
@@ -589,15 +612,21 @@ RustError scan_view(Fr* base, uint64_t s, uint64_t m, const Fr* prefix, hipStrea
 constexpr unsigned kEwBlock = 256;
 unsigned ew_grid(uint64_t n) { return grid_for(n, kEwBlock, 256u * 16u); }
 
+// The element-wise kernels below run on the 29-bit form: stored words (x * 2^256)
+// load as normalised limbs; the product of two stored words is
+// mul(shl5(a), b) (one operand moved to the 2^261 scale by a 5-bit shift, which is
+// cheaper than a product); sums of two canonical words (< 2m, limbs < 2^30) enter a
+// product as they are; results are reduced to canonical words before they are stored.
 // den[i] *= beta * perm[i] + gamma + value[i]        (permutation_commit.h:12-38)
 __global__ void perm_denominator_kernel(Fr* __restrict__ den, const Fr* __restrict__ perm, const Fr* __restrict__ val,
     const Fr* __restrict__ d_beta, const Fr* __restrict__ d_gamma, uint64_t n)
 {
-    const Fr beta = Fr::load(d_beta), gamma = Fr::load(d_gamma);
+    const R29 beta32 = R29::shl5(ld29(d_beta)), gamma = ld29(d_gamma);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        Fr t = beta * Fr::load(perm + i) + gamma + Fr::load(val + i);
-        (Fr::load(den + i) * t).store(den + i);
+        // beta * perm (normalised, < 1.01m) + gamma + value: < 3.01m, limbs < 2^31
+        const R29 t = R29::add(R29::add(R29::mul(beta32, ld29(perm + i)), gamma), ld29(val + i));
+        st29(R29::mul(R29::shl5(ld29(den + i)), t), den + i);
     }
 }
 
@@ -626,16 +655,17 @@ __global__ void perm_numerator_kernel(Fr* __restrict__ num, const Fr* __restrict
     const Fr* __restrict__ d_gamma, const Fr* __restrict__ lo, const Fr* __restrict__ hi,
     const Fr* __restrict__ d_deltaomega, uint64_t n, int first)
 {
-    const Fr gamma = Fr::load(d_gamma);
-    const Fr c = Fr::load(d_deltaomega) * Fr::load(d_beta);
+    const R29 gamma = ld29(d_gamma);
+    // (deltaomega * beta) in the 2^261 scale
+    const R29 c32 = R29::shl5(R29::mul(R29::shl5(ld29(d_deltaomega)), ld29(d_beta)));
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const Fr w = Fr::load(lo + (i & ((1u << kSplit) - 1))) * Fr::load(hi + (i >> kSplit));
-        Fr t = c * w + gamma + Fr::load(val + i);
+        const R29 w = R29::mul(R29::shl5(ld29(lo + (i & ((1u << kSplit) - 1)))), ld29(hi + (i >> kSplit)));
+        const R29 t = R29::add(R29::add(R29::mul(c32, w), gamma), ld29(val + i)); // < 3.01m
         if (first)
-            t.store(num + i);
+            st29(t, num + i);
         else
-            (Fr::load(num + i) * t).store(num + i);
+            st29(R29::mul(R29::shl5(ld29(num + i)), t), num + i);
     }
 }
 
@@ -643,7 +673,7 @@ __global__ void mul_into_kernel(Fr* __restrict__ a, const Fr* __restrict__ b, ui
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        (Fr::load(a + i) * Fr::load(b + i)).store(a + i);
+        st29(R29::mul(R29::shl5(ld29(a + i)), ld29(b + i)), a + i);
 }
 
 __global__ void scalar_mul_kernel(Fr* __restrict__ a, const Fr* __restrict__ b)
@@ -656,21 +686,24 @@ __global__ void scalar_mul_kernel(Fr* __restrict__ a, const Fr* __restrict__ b)
 __global__ void lookup_denominator_kernel(Fr* __restrict__ out, const Fr* __restrict__ pi, const Fr* __restrict__ pt,
     const Fr* __restrict__ d_beta, const Fr* __restrict__ d_gamma, uint64_t n)
 {
-    const Fr beta = Fr::load(d_beta), gamma = Fr::load(d_gamma);
+    const R29 beta32 = R29::shl5(ld29(d_beta)), gamma = ld29(d_gamma);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        ((Fr::load(pi + i) + beta) * (Fr::load(pt + i) + gamma)).store(out + i);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        // 32 (A' + beta) < 64m and S' + gamma < 2m: both limb sets < 2^30
+        const R29 x = R29::add(R29::shl5(ld29(pi + i)), beta32);
+        st29(R29::mul(x, R29::add(ld29(pt + i), gamma)), out + i);
+    }
 }
 
 // out *= (A + beta)(S + gamma)                        (commit_product.h:40-66)
 __global__ void lookup_numerator_kernel(Fr* __restrict__ out, const Fr* __restrict__ ci, const Fr* __restrict__ ct,
     const Fr* __restrict__ d_beta, const Fr* __restrict__ d_gamma, uint64_t n)
 {
-    const Fr beta = Fr::load(d_beta), gamma = Fr::load(d_gamma);
+    const R29 beta = ld29(d_beta), gamma = ld29(d_gamma);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        Fr v = Fr::load(out + i) * (Fr::load(ci + i) + beta);
-        (v * (Fr::load(ct + i) + gamma)).store(out + i);
+        const R29 v = R29::mul(R29::shl5(ld29(out + i)), R29::add(ld29(ci + i), beta)); // < 1.01m, normalised
+        st29(R29::mul(R29::shl5(v), R29::add(ld29(ct + i), gamma)), out + i);
     }
 }
 
