@@ -379,6 +379,7 @@ def cpu_measurements(msm, dv, F, k: int, max_threads: int):
     a16 = co.chacha_fr(31, 0, 1 << 16)
     a20 = co.chacha_fr(32, 0, 1 << 20)
     host["ntt_coset_2^16_elems_per_s"] = (1 << 16) / _median_time(lambda: coset_ntt(a16, 16, threads), 5)
+    host["ntt_coset_2^20_elems_per_s"] = (1 << 20) / _median_time(lambda: coset_ntt(a20, 20, threads), 5)
     host["ntt_coset_2^20_1thread_elems_per_s"] = (1 << 20) / _median_time(lambda: coset_ntt(a20, 20, 1), 3)
     for lk in (16, 20):
         m = 1 << lk
@@ -387,6 +388,17 @@ def cpu_measurements(msm, dv, F, k: int, max_threads: int):
         host[f"msm_2^{lk}_elems_per_s"] = m / _median_time(lambda: co.msm(s_, pts, threads), 5)
         if lk == 16:
             host["msm_2^16_1thread_elems_per_s"] = m / _median_time(lambda: co.msm(s_, pts, 1), 3)
+    # The k = 20 proof on the CPU, estimated from these rates (no CPU prover runs here):
+    # its MSMs (5 advice, 2 permuted lookup columns, 2 permutation sets + 1 lookup
+    # product, 3 quotient pieces, 2 SHPLONK commitments: 15 of 2^20 points) and its NTTs
+    # (11 inverse transforms to coefficients, 4 extended parts x 11 coset transforms, the
+    # 2^22 inverse of h: ~59 transforms of 2^20 elements).
+    msm_s = 15 * (1 << 20) / host["msm_2^20_elems_per_s"]
+    ntt_s = 59 * (1 << 20) / host["ntt_coset_2^20_elems_per_s"]
+    host["proof_k20_estimate_s"] = round(msm_s + ntt_s, 2)
+    host["proof_k20_estimate_note"] = ("estimate, not a run: 15 MSMs of 2^20 points and ~59 NTTs of 2^20 "
+                                       "elements at the measured CPU rates; quotient, products and opening "
+                                       "arithmetic left out")
     return cpu, host
 
 

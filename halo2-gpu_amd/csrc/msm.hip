@@ -1062,7 +1062,11 @@ __global__ void __launch_bounds__(256) reduce_first_kernel(const PointSlot* __re
 // S scaled by K); adjacent blocks of equal size then combine as
 // T = T_L + T_R + S_R, S = 2 (S_L + S_R); after log2(cnt) rounds the
 // block's (T, S) is in the same form with cnt * K times larger segments.
-__global__ void __launch_bounds__(TREE) reduce_tree_kernel(const PointSlot* __restrict__ Ti,
+// The rounds are a chain of dependent point operations on few lanes (latency,
+// not throughput, bounds them), so the two halves of a combine run on different
+// waves: threads [0, TREE) form T, threads [TREE, 2 TREE) form S of the same node,
+// which halves the chain (2 point operations per round instead of 4).
+__global__ void __launch_bounds__(2 * TREE) reduce_tree_kernel(const PointSlot* __restrict__ Ti,
     const PointSlot* __restrict__ Si, uint64_t Mi, uint32_t K, uint32_t log_k, PointSlot* __restrict__ To,
     PointSlot* __restrict__ So)
 {
@@ -1073,9 +1077,10 @@ __global__ void __launch_bounds__(TREE) reduce_tree_kernel(const PointSlot* __re
     const uint32_t cnt = (uint32_t)(per_blk / K);
     const uint64_t w = blockIdx.x / Mo, g = blockIdx.x % Mo;
     const uint64_t base = w * Mi + g * per_blk;
-    const uint32_t i = threadIdx.x;
+    const bool s_role = threadIdx.x >= TREE; // whole waves: TREE is a multiple of 64
+    const uint32_t i = s_role ? threadIdx.x - TREE : threadIdx.x;
     const bool keep_s = cnt > 1 || Mo > 1;
-    if (i < cnt) {
+    if (!s_role && i < cnt) {
         G1X29 a = G1X29::inf(), run = G1X29::inf(), acc = G1X29::inf();
         for (int j = (int)K - 1; j >= 0; --j) {
             const uint64_t q = base + (uint64_t)i * K + (uint32_t)j;
@@ -1090,23 +1095,25 @@ __global__ void __launch_bounds__(TREE) reduce_tree_kernel(const PointSlot* __re
     }
     for (uint32_t d = 1; d < cnt; d <<= 1) {
         __syncthreads();
-        G1X29 t, s;
-        const bool act = (i & (2 * d - 1)) == 0 && i + d < cnt;
+        G1X29 v;
         const bool need_s = 2 * d < cnt || Mo > 1;
+        const bool act = (i & (2 * d - 1)) == 0 && i + d < cnt && (!s_role || need_s);
         if (act) {
-            t = G1X29::add(G1X29::add(sT[i], sT[i + d]), sS[i + d]);
-            if (need_s)
-                s = G1X29::dbl(G1X29::add(sS[i], sS[i + d]));
+            if (!s_role)
+                v = G1X29::add(G1X29::add(sT[i], sT[i + d]), sS[i + d]);
+            else
+                v = G1X29::dbl(G1X29::add(sS[i], sS[i + d]));
         }
         __syncthreads();
         if (act) {
-            sT[i] = t;
-            if (need_s)
-                sS[i] = s;
+            if (!s_role)
+                sT[i] = v;
+            else
+                sS[i] = v;
         }
     }
     __syncthreads();
-    if (i == 0) {
+    if (threadIdx.x == 0) {
         sT[0].store(To + blockIdx.x);
         if (Mo > 1)
             sS[0].store(So + blockIdx.x);
@@ -1372,7 +1379,7 @@ RustError msm_device(const MsmPlan& p, const MsmBuffers& b, const Fr* d_scalars,
     for (uint32_t lev = 1; lev <= p.nlev; ++lev) {
         const uint64_t Mi = p.Mlev[lev - 1];
         const uint64_t Mo = p.Mlev[lev];
-        reduce_tree_kernel<<<(unsigned)(p.Wb * Mo), TREE, 0, stream>>>(b.red[2 * cur], b.red[2 * cur + 1], Mi,
+        reduce_tree_kernel<<<(unsigned)(p.Wb * Mo), 2 * TREE, 0, stream>>>(b.red[2 * cur], b.red[2 * cur + 1], Mi,
             p.Klev[lev], ilog2(p.Klev[lev]), b.red[2 * (cur ^ 1)], b.red[2 * (cur ^ 1) + 1]);
         H2_LAUNCH_CHECK();
         cur ^= 1;

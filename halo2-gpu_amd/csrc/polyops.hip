@@ -14,6 +14,7 @@
 // every output of a run ending at e.
 #include "common.h"
 #include "field.h"
+#include "field29.h"
 
 using namespace h2;
 
@@ -249,8 +250,11 @@ unsigned ew_blocks(uint64_t n)
 
 // --------------------------------------------------------- kate division
 
-constexpr uint64_t kRun = 64; // outputs per thread in one recurrence level
-constexpr int kMaxLevels = 12; // 64^12 > 2^64
+// outputs per thread in one recurrence level: each level is a chain of kRun
+// dependent products on one lane, so short runs (more threads, one more level at
+// 2^20) cut the latency that bounds the division
+constexpr uint64_t kRun = 32;
+constexpr int kMaxLevels = 13; // 32^13 > 2^64
 
 struct KateLevel {
     uint64_t len; // outputs at this level
@@ -305,24 +309,27 @@ __global__ void kate_powers_kernel(const Fr* __restrict__ d_u, Fr* __restrict__ 
 // Local solve of q_j = src(j + 1) + u q_{j + 1} on run r = [r kRun, e) with a
 // zero boundary q_e = 0: writes loc_j for j < cap into q and the run head
 // loc_{r kRun} into heads[r]. src(i) is read for i in 1..len.
+// The recurrence runs on the 29-bit form (field29.h): u enters in the 2^261 scale
+// (a 5-bit shift of its stored word), so mul(loc, u) is the stored word of loc * u;
+// loc stays below 2.01m (a normalised product below 1.01m plus a canonical word).
 __global__ void kate_local_kernel(const Fr* __restrict__ src, Fr* __restrict__ q, uint64_t len, uint64_t cap,
     uint64_t runs, const Fr* __restrict__ pw, Fr* __restrict__ heads)
 {
     const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= runs)
         return;
-    const Fr u = Fr::load(pw + 1);
+    const Fr29 u = Fr29::shl5(Fr29::load32(pw + 1));
     const uint64_t lo = r * kRun;
     const uint64_t e = lo + kRun < len ? lo + kRun : len;
-    Fr loc = Fr::load(src + e);
+    Fr29 loc = Fr29::load32(src + e);
     if (e - 1 < cap)
-        loc.store(q + e - 1);
+        Fr29::reduce_full(loc).store32(q + e - 1);
     for (uint64_t j = e - 1; j-- > lo;) {
-        loc = Fr::load(src + j + 1) + u * loc;
+        loc = Fr29::add(Fr29::load32(src + j + 1), Fr29::mul(loc, u));
         if (j < cap)
-            loc.store(q + j);
+            Fr29::reduce_full(Fr29::normalize(loc)).store32(q + j);
     }
-    loc.store(heads + r);
+    Fr29::reduce_full(Fr29::normalize(loc)).store32(heads + r);
 }
 
 // With Q_s the true value at the head of run s: q_j += u^(e - j) Q_{r + 1},
@@ -337,7 +344,8 @@ __global__ void kate_fix_kernel(Fr* __restrict__ q, uint64_t cap, uint64_t runs,
     if (r + 1 >= runs)
         return;
     const uint64_t e = (r + 1) * kRun;
-    (Fr::load(q + j) + Fr::load(pw + (e - j)) * Fr::load(solved + r + 1)).store(q + j);
+    const Fr29 t = Fr29::mul(Fr29::shl5(Fr29::load32(pw + (e - j))), Fr29::load32(solved + r + 1));
+    Fr29::reduce_full(Fr29::normalize(Fr29::add(Fr29::load32(q + j), t))).store32(q + j);
 }
 
 // q_j = src(j + 1) + u q_{j + 1} for j < len with q_len = 0, written for j < cap.
