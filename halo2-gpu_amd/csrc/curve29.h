@@ -194,28 +194,4 @@ struct G1X29 {
     }
 };
 
-// Jacobian doubling for long doubling chains (the Horner over MSM windows):
-// dbl-2009-l for a = 0, 2M + 5S against XYZZ's 6M + 3S. Coordinates < 2m,
-// normalised, in and out; Z = 0 is the identity and stays it.
-__device__ __forceinline__ void jacobian_dbl(Fq29& X, Fq29& Y, Fq29& Z)
-{
-    const Fq29 A = Fq29::sqr(X); // < 1.01
-    const Fq29 B = Fq29::sqr(Y);
-    const Fq29 C = Fq29::sqr(B);
-    const Fq29 T = Fq29::sqr(Fq29::add(X, B)); // (X + B)^2, operand < 3.01, limbs < 2^30
-    const Fq29 AC = Fq29::normalize(Fq29::add(A, C)); // < 2.02
-    const Fq29 Dh = Fq29::reduce_2m(Fq29::normalize(Fq29::sub4(T, AC))); // (X + B)^2 - A - C, < 2
-    const Fq29 D = Fq29::reduce_2m(Fq29::normalize(Fq29::add(Dh, Dh))); // < 2
-    const Fq29 E = Fq29::normalize(Fq29::add(Fq29::add(A, A), A)); // 3A < 3.03
-    const Fq29 F = Fq29::sqr(E); // < 1.01
-    const Fq29 D2 = Fq29::normalize(Fq29::add(D, D)); // < 4
-    const Fq29 X3 = Fq29::reduce_2m(Fq29::normalize(Fq29::sub4(F, D2))); // F - 2D, < 2
-    const Fq29 t = Fq29::mul(E, nsub2(D, X3)); // E (D - X3), < 1.01
-    const Fq29 C2 = Fq29::add(C, C);
-    const Fq29 C8 = Fq29::reduce_2m(Fq29::normalize(Fq29::add(Fq29::add(C2, C2), Fq29::add(C2, C2)))); // 8C, < 2
-    Z = Fq29::mul(Fq29::add(Y, Y), Z); // 2 Y Z: operand limbs < 2^30, < 1.01
-    Y = Fq29::reduce_2m(nsub2(t, C8)); // E (D - X3) - 8C: < 3.01 -> < 2
-    X = X3;
-}
-
 } // namespace h2

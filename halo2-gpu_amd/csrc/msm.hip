@@ -27,7 +27,9 @@
 //     trees over 256 pairs keep that form (T = T_L + T_R + S_R,
 //     S = 2 (S_L + S_R), S carrying the block size as a weight) until one
 //     pair per window is left.
-//  6. Horner over windows (c doublings per window), conversion to affine.
+//  6. on the host: Horner over the exported window sums (c doublings per
+//     window) and conversion to affine, a serial chain a CPU core finishes
+//     faster than one GPU lane.
 // The result is written as (x, y, 1) or the identity (0, 1, 0) in Montgomery
 // form, the layout of the reference's `cukernel_to_affine` output.
 #include <atomic>
@@ -304,7 +306,7 @@ void carve(WS& ws, const MsmPlan& p, MsmBuffers* b)
     for (int i = 0; i < 4; ++i)
         v[k++] = ws.take(128 * nred);
     v[k++] = ws.take(scan_u32_scratch(scan_n));
-    v[k++] = ws.take(96ull * p.J);
+    v[k++] = ws.take(128ull * p.Wb); // window sums exported to the host
     if (b) {
         k = 0;
         b->pts = (uint8_t*)v[k++];
@@ -1120,36 +1122,231 @@ __global__ void __launch_bounds__(2 * TREE) reduce_tree_kernel(const PointSlot* 
     }
 }
 
-// One thread per MSM: Horner over its W window sums (win[j * W + w]).
-__global__ void __launch_bounds__(64) finalize_kernel(const PointSlot* __restrict__ win, uint32_t W, uint32_t c,
-    uint32_t J, uint64_t* __restrict__ out)
+// The window sums leave the device as XYZZ points in the boundary's 2^256 scale
+// (4 x 32 B each, out[t] for window sum t < Wb): the Horner over windows (a chain
+// of c (W - 1) doublings) and the affine conversion run on the host, where one
+// core finishes the chain in ~0.1 ms while a single GPU lane needs ~1.5 ms for it.
+__global__ void __launch_bounds__(64) export_windows_kernel(const PointSlot* __restrict__ win, uint32_t Wb,
+    uint64_t* __restrict__ out)
 {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= J)
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= Wb)
         return;
-    win += (uint64_t)j * W;
-    out += 12ull * j;
-    // the c doublings per window run in Jacobian coordinates (7 products per
-    // doubling instead of XYZZ's 9), the window additions in XYZZ
-    Fq29 X, Y, Z;
-    G1X29::load(win + (W - 1)).to_jacobian(X, Y, Z);
-    for (int w = (int)W - 2; w >= 0; --w) {
-        for (uint32_t k = 0; k < c; ++k)
-            jacobian_dbl(X, Y, Z);
-        G1X29::add(G1X29::from_jacobian(X, Y, Z), G1X29::load(win + w)).to_jacobian(X, Y, Z);
+    const G1Xyzz x = G1X29::load(win + t).to_xyzz256();
+    x.X.store(out + 16ull * t);
+    x.Y.store(out + 16ull * t + 4);
+    x.ZZ.store(out + 16ull * t + 8);
+    x.ZZZ.store(out + 16ull * t + 12);
+}
+
+// ------------------------------------------------ host side: window Horner
+//
+// Fq in 4 x 64-bit Montgomery form (R = 2^256, the boundary's representation),
+// CIOS multiplication; XYZZ points (add-2008-s, dbl-2008-s-1, a = 0) as on the
+// device.
+struct HFq {
+    uint64_t v[4];
+};
+
+constexpr uint64_t kHP[4] = { 0x3c208c16d87cfd47ull, 0x97816a916871ca8dull, 0xb85045b68181585dull,
+    0x30644e72e131a029ull };
+constexpr uint64_t kHN0 = 0x87d20782e4866389ull; // -p^-1 mod 2^64
+constexpr uint64_t kHOne[4] = { 0xd35d438dc58f0d9dull, 0x0a78eb28f5c70b3dull, 0x666ea36f7879462cull,
+    0x0e0a77c19a07df2full }; // 2^256 mod p
+
+bool hfq_geq_p(const uint64_t (&a)[4])
+{
+    for (int i = 3; i >= 0; --i)
+        if (a[i] != kHP[i])
+            return a[i] > kHP[i];
+    return true;
+}
+
+HFq hfq_sub_p(const HFq& a)
+{
+    HFq r;
+    unsigned __int128 b = 0;
+    for (int i = 0; i < 4; ++i) {
+        const unsigned __int128 d = (unsigned __int128)a.v[i] - kHP[i] - (uint64_t)b;
+        r.v[i] = (uint64_t)d;
+        b = (d >> 64) ? 1 : 0;
     }
-    const G1X29 acc = G1X29::from_jacobian(X, Y, Z);
-    Fq one = Fq::one();
-    if (acc.is_inf()) {
-        Fq::zero().store(out);
-        one.store(out + 4);
-        Fq::zero().store(out + 8);
-    } else {
-        G1Affine a = acc.to_xyzz256().to_affine();
-        a.x.store(out);
-        a.y.store(out + 4);
-        one.store(out + 8);
+    return r;
+}
+
+HFq hfq_add(const HFq& a, const HFq& b)
+{
+    HFq r;
+    unsigned __int128 c = 0;
+    for (int i = 0; i < 4; ++i) {
+        c += (unsigned __int128)a.v[i] + b.v[i];
+        r.v[i] = (uint64_t)c;
+        c >>= 64;
     }
+    return (c || hfq_geq_p(r.v)) ? hfq_sub_p(r) : r; // a + b < 2p < 2^256
+}
+
+HFq hfq_sub(const HFq& a, const HFq& b)
+{
+    HFq r;
+    unsigned __int128 br = 0;
+    for (int i = 0; i < 4; ++i) {
+        const unsigned __int128 d = (unsigned __int128)a.v[i] - b.v[i] - (uint64_t)br;
+        r.v[i] = (uint64_t)d;
+        br = (d >> 64) ? 1 : 0;
+    }
+    if (br) { // add p back
+        unsigned __int128 c = 0;
+        for (int i = 0; i < 4; ++i) {
+            c += (unsigned __int128)r.v[i] + kHP[i];
+            r.v[i] = (uint64_t)c;
+            c >>= 64;
+        }
+    }
+    return r;
+}
+
+HFq hfq_mul(const HFq& a, const HFq& b)
+{
+    uint64_t t[6] = { 0, 0, 0, 0, 0, 0 };
+    for (int i = 0; i < 4; ++i) {
+        unsigned __int128 c = 0;
+        for (int j = 0; j < 4; ++j) {
+            c += (unsigned __int128)a.v[j] * b.v[i] + t[j];
+            t[j] = (uint64_t)c;
+            c >>= 64;
+        }
+        c += t[4];
+        t[4] = (uint64_t)c;
+        t[5] = (uint64_t)(c >> 64);
+        const uint64_t m = t[0] * kHN0;
+        c = ((unsigned __int128)m * kHP[0] + t[0]) >> 64;
+        for (int j = 1; j < 4; ++j) {
+            c += (unsigned __int128)m * kHP[j] + t[j];
+            t[j - 1] = (uint64_t)c;
+            c >>= 64;
+        }
+        c += t[4];
+        t[3] = (uint64_t)c;
+        t[4] = t[5] + (uint64_t)(c >> 64);
+    }
+    HFq r { { t[0], t[1], t[2], t[3] } };
+    return (t[4] || hfq_geq_p(r.v)) ? hfq_sub_p(r) : r;
+}
+
+bool hfq_is_zero(const HFq& a) { return (a.v[0] | a.v[1] | a.v[2] | a.v[3]) == 0; }
+
+bool hfq_eq(const HFq& a, const HFq& b)
+{
+    return a.v[0] == b.v[0] && a.v[1] == b.v[1] && a.v[2] == b.v[2] && a.v[3] == b.v[3];
+}
+
+HFq hfq_inv(const HFq& a) // a^(p - 2); Montgomery in, Montgomery out
+{
+    uint64_t e[4] = { kHP[0] - 2, kHP[1], kHP[2], kHP[3] };
+    HFq r { { kHOne[0], kHOne[1], kHOne[2], kHOne[3] } };
+    for (int i = 255; i >= 0; --i) {
+        r = hfq_mul(r, r);
+        if ((e[i >> 6] >> (i & 63)) & 1)
+            r = hfq_mul(r, a);
+    }
+    return r;
+}
+
+struct HXyzz {
+    HFq X, Y, ZZ, ZZZ; // ZZ = 0: the identity
+};
+
+HXyzz hx_dbl(const HXyzz& p)
+{
+    if (hfq_is_zero(p.ZZ))
+        return p;
+    const HFq U = hfq_add(p.Y, p.Y);
+    const HFq V = hfq_mul(U, U);
+    const HFq W = hfq_mul(U, V);
+    const HFq S = hfq_mul(p.X, V);
+    const HFq X2 = hfq_mul(p.X, p.X);
+    const HFq M = hfq_add(hfq_add(X2, X2), X2);
+    HXyzz r;
+    r.X = hfq_sub(hfq_mul(M, M), hfq_add(S, S));
+    r.Y = hfq_sub(hfq_mul(M, hfq_sub(S, r.X)), hfq_mul(W, p.Y));
+    r.ZZ = hfq_mul(V, p.ZZ);
+    r.ZZZ = hfq_mul(W, p.ZZZ);
+    return r;
+}
+
+HXyzz hx_add(const HXyzz& p, const HXyzz& q)
+{
+    if (hfq_is_zero(q.ZZ))
+        return p;
+    if (hfq_is_zero(p.ZZ))
+        return q;
+    const HFq U1 = hfq_mul(p.X, q.ZZ), U2 = hfq_mul(q.X, p.ZZ);
+    const HFq S1 = hfq_mul(p.Y, q.ZZZ), S2 = hfq_mul(q.Y, p.ZZZ);
+    const HFq P = hfq_sub(U2, U1), R = hfq_sub(S2, S1);
+    if (hfq_is_zero(P)) {
+        if (hfq_is_zero(R))
+            return hx_dbl(p);
+        return HXyzz {}; // ZZ = 0: the identity
+    }
+    const HFq PP = hfq_mul(P, P), PPP = hfq_mul(P, PP), Q = hfq_mul(U1, PP);
+    HXyzz r;
+    r.X = hfq_sub(hfq_sub(hfq_mul(R, R), PPP), hfq_add(Q, Q));
+    r.Y = hfq_sub(hfq_mul(R, hfq_sub(Q, r.X)), hfq_mul(S1, PPP));
+    r.ZZ = hfq_mul(hfq_mul(p.ZZ, q.ZZ), PP);
+    r.ZZZ = hfq_mul(hfq_mul(p.ZZZ, q.ZZZ), PPP);
+    return r;
+}
+
+HXyzz hx_load(const uint64_t* w)
+{
+    HXyzz r;
+    for (int i = 0; i < 4; ++i) {
+        r.X.v[i] = w[i];
+        r.Y.v[i] = w[4 + i];
+        r.ZZ.v[i] = w[8 + i];
+        r.ZZZ.v[i] = w[12 + i];
+    }
+    return r;
+}
+
+// J results of 96 B ((x, y, 1) or (0, 1, 0), Montgomery) from J x Wj window sums
+// of 128 B: sum over w of 2^(c w) S_w by Horner.
+void host_finalize(const uint64_t* win, uint32_t Wj, uint32_t c, uint32_t J, uint64_t* out)
+{
+    for (uint32_t j = 0; j < J; ++j) {
+        const uint64_t* wj = win + 16ull * Wj * j;
+        HXyzz acc = hx_load(wj + 16ull * (Wj - 1));
+        for (int w = (int)Wj - 2; w >= 0; --w) {
+            for (uint32_t k = 0; k < c; ++k)
+                acc = hx_dbl(acc);
+            acc = hx_add(acc, hx_load(wj + 16ull * w));
+        }
+        uint64_t* o = out + 12ull * j;
+        memset(o, 0, 96);
+        if (hfq_is_zero(acc.ZZ)) {
+            memcpy(o + 4, kHOne, 32);
+            continue;
+        }
+        const HFq inv = hfq_inv(hfq_mul(acc.ZZ, acc.ZZZ)); // 1 / (ZZ ZZZ)
+        const HFq x = hfq_mul(acc.X, hfq_mul(acc.ZZZ, inv)); // X / ZZ
+        const HFq y = hfq_mul(acc.Y, hfq_mul(acc.ZZ, inv)); // Y / ZZZ
+        memcpy(o, x.v, 32);
+        memcpy(o + 4, y.v, 32);
+        memcpy(o + 8, kHOne, 32);
+    }
+}
+
+// Copies the J x Wj window sums of a finished pipeline to the host, waits, and
+// writes the J 96-byte results to `h_out`.
+RustError finish_on_host(const MsmPlan& p, const MsmBuffers& b, void* h_out, hipStream_t stream)
+{
+    const uint32_t Wj = p.Wb / p.J;
+    std::vector<uint64_t> win(16ull * p.Wb);
+    H2_TRY(hipMemcpyAsync(win.data(), b.out, 128ull * p.Wb, hipMemcpyDeviceToHost, stream));
+    H2_TRY(hipStreamSynchronize(stream));
+    host_finalize(win.data(), Wj, p.c, p.J, (uint64_t*)h_out);
+    return ok();
 }
 
 // ------------------------------------------------- precomputed window bases
@@ -1310,7 +1507,7 @@ unsigned blocks_for(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t)
 
 uint32_t ilog2(uint32_t x) { return 31 - __builtin_clz(x); }
 
-// Full device pipeline; the 96-byte results land in b.out (device). A batch
+// Full device pipeline; the window sums land in b.out (device, for finish_on_host). A batch
 // (p.J > 1) has its canonical scalars and job tables in place already and
 // passes d_scalars = null.
 RustError msm_device(const MsmPlan& p, const MsmBuffers& b, const Fr* d_scalars, const uint8_t* d_points,
@@ -1384,7 +1581,7 @@ RustError msm_device(const MsmPlan& p, const MsmBuffers& b, const Fr* d_scalars,
         H2_LAUNCH_CHECK();
         cur ^= 1;
     }
-    finalize_kernel<<<blocks_for(p.J, 64), 64, 0, stream>>>(b.red[2 * cur], p.Wb / p.J, p.c, p.J, b.out);
+    export_windows_kernel<<<blocks_for(p.Wb, 64), 64, 0, stream>>>(b.red[2 * cur], p.Wb, b.out);
     H2_LAUNCH_CHECK();
     return ok();
 }
@@ -1424,9 +1621,7 @@ RustError msm_entry(const char* who, const void* scalars, bool scalars_on_device
     RustError e = msm_device(p, b, d_scal, d_pts, stream);
     if (e.code)
         return e;
-    H2_TRY(hipMemcpyAsync(h_out, b.out, 96, hipMemcpyDeviceToHost, stream));
-    H2_TRY(hipStreamSynchronize(stream));
-    return ok();
+    return finish_on_host(p, b, h_out, stream);
 }
 
 } // namespace
@@ -1611,9 +1806,7 @@ RustError h2amd_multiexp_precomputed(const void* d_scalar, const void* d_table, 
     RustError e = msm_device(p, b, (const Fr*)d_scalar, (const uint8_t*)d_table, stream);
     if (e.code)
         return e;
-    H2_TRY(hipMemcpyAsync(o, b.out, 96, hipMemcpyDeviceToHost, stream));
-    H2_TRY(hipStreamSynchronize(stream));
-    return ok();
+    return finish_on_host(p, b, o, stream);
 }
 
 // ------------------------------------------ batched MSMs (no reference counterpart)
@@ -1681,10 +1874,8 @@ static RustError batch_entry(const char* who, const void* const* d_scalars, cons
     RustError e = msm_device(p, b, nullptr, (const uint8_t*)d_bases, stream);
     if (e.code)
         return e;
-    H2_TRY(hipMemcpyAsync(o, b.out, 96ull * count, hipMemcpyDeviceToHost, stream));
-    // also keeps the host job tables alive until their copies are done
-    H2_TRY(hipStreamSynchronize(stream));
-    return ok();
+    // its synchronisation also keeps the host job tables alive until their copies are done
+    return finish_on_host(p, b, o, stream);
 }
 
 RustError h2amd_multiexp_batch(const void* const* d_scalars, const uint64_t* lengths, uint32_t count,
