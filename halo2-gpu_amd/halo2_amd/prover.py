@@ -401,9 +401,11 @@ def rotate(dom: EvaluationDomain, x: int, rot: int) -> int:
     return x * pow(w, abs(rot), R_MOD) % R_MOD
 
 
-def create_proof(params: msm.ParamsKZG, pk: ProvingKey, circuit: Circuit, seed: int = 1) -> dict:
+def create_proof(params: msm.ParamsKZG, pk: ProvingKey, circuit: Circuit, seed: int = 1, phase_hook=None) -> dict:
     """One proof; returns the proof bytes, the public values, the per-phase times (ms,
-    synchronised on this thread's stream) and the self-checks."""
+    synchronised on this thread's stream) and the self-checks. `phase_hook(name)`, when
+    given, runs after each phase has finished on the device (a serving loop uses it to
+    let smaller proofs run first)."""
     lib = _lib.load()
     cs = circuit.cs
     dom, n, u = pk.domain, 1 << circuit.k, circuit.usable
@@ -417,7 +419,9 @@ def create_proof(params: msm.ParamsKZG, pk: ProvingKey, circuit: Circuit, seed: 
         torch.cuda.current_stream().synchronize()  # this proof's stream only: proofs may run side by side
         t = time.perf_counter()
         times[name] = round((t - t0) * 1e3, 3)
-        t0 = t
+        if phase_hook is not None:
+            phase_hook(name)
+        t0 = time.perf_counter()
 
     # 0. the key's digest and the instance values (prover.rs:165; witness.rs:237-245)
     tr.common_scalar(pk.vk.transcript_repr)

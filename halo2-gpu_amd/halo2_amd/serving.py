@@ -80,9 +80,17 @@ def prepare_setup(k: int, seed: int, tau: int, table_bits: int = 16, fixed_base:
 
 
 class ProvingService:
-    def __init__(self, setups: dict, slots: int = 4, hbm_budget: int | None = None, reserve: int = 4 << 30):
+    """`yield_ms`: between its phases, a proof waits (at most this long per phase) while a
+    proof of a smaller circuit is running and no request is waiting, so a short proof
+    that arrives during a long one is not stretched by sharing the device with it. The
+    pause applies only while the service keeps up: under a backlog every proof runs
+    flat out, since pausing then only delays the queue. 0 disables."""
+
+    def __init__(self, setups: dict, slots: int = 4, hbm_budget: int | None = None, reserve: int = 4 << 30,
+                 yield_ms: float = 0.0):
         self.setups = setups
         self.slots = slots
+        self.yield_s = yield_ms / 1e3
         if hbm_budget is None:
             total = torch.cuda.get_device_properties(torch.cuda.current_device()).total_memory
             hbm_budget = total - torch.cuda.memory_allocated() - reserve
@@ -91,6 +99,7 @@ class ProvingService:
             if s.footprint > self.budget:
                 raise ValueError(f"a proof at k={k} needs {s.footprint} bytes, more than the budget {self.budget}")
         self.cv = threading.Condition()
+        self._running_k: dict = {}  # id(request) -> k of the proofs on the device
         self.waiting: deque = deque()
         self.in_flight = 0
         self.in_flight_bytes = 0
@@ -125,6 +134,7 @@ class ProvingService:
             need = self.setups[req.k].footprint
             if self.in_flight_bytes + need <= self.budget:
                 del self.waiting[i]
+                self._running_k[id(req)] = req.k
                 self.in_flight += 1
                 self.in_flight_bytes += need
                 self.max_in_flight = max(self.max_in_flight, self.in_flight)
@@ -144,20 +154,33 @@ class ProvingService:
                     req = self._take()
             s = self.setups[req.k]
             req.start = self.now()
+            hook = (lambda _name, r=req: self._yield_to_smaller(r)) if self.yield_s > 0 else None
             try:
                 with torch.cuda.stream(stream):
                     circuit = prover.fresh_witness(s.circuit, req.witness_seed)
-                    req.result = prover.create_proof(s.params, s.pk, circuit, seed=req.blind_seed)
+                    req.result = prover.create_proof(s.params, s.pk, circuit, seed=req.blind_seed, phase_hook=hook)
                     del circuit
                 stream.synchronize()
             except BaseException as e:  # reported on the request; the service keeps running
                 req.error = e
             req.done = self.now()
             with self.cv:
+                self._running_k.pop(id(req), None)
                 self.in_flight -= 1
                 self.in_flight_bytes -= s.footprint
                 self.cv.notify_all()
             req.event.set()
+
+    def _yield_to_smaller(self, req: Request) -> None:
+        """Between two phases of `req`: wait while a smaller circuit's proof runs and
+        nothing waits for a slot (bounded)."""
+        deadline = time.perf_counter() + self.yield_s
+        with self.cv:
+            while not self.waiting and any(k < req.k for key, k in self._running_k.items() if key != id(req)):
+                left = deadline - time.perf_counter()
+                if left <= 0:
+                    return
+                self.cv.wait(left)
 
 
 def poisson_trace(seconds: float, rate: float, ks, weights, seed: int) -> list:
@@ -172,13 +195,13 @@ def poisson_trace(seconds: float, rate: float, ks, weights, seed: int) -> list:
 
 
 def run_trace(setups: dict, trace: list, slots: int, seed: int = 0, hbm_budget: int | None = None,
-              progress=None, progress_every_s: float = 30.0) -> dict:
+              progress=None, progress_every_s: float = 30.0, yield_ms: float = 0.0) -> dict:
     """Open-loop run: every request is submitted at its arrival time whatever the
     service's state; each gets a fresh witness (seed + index) and blinders. Returns
     throughput, latency percentiles (arrival to finished proof), per-k latencies and the
     peak number of proofs in flight. `progress(message)`, when given, is called about
     every `progress_every_s` seconds of the run."""
-    svc = ProvingService(setups, slots=slots, hbm_budget=hbm_budget)
+    svc = ProvingService(setups, slots=slots, hbm_budget=hbm_budget, yield_ms=yield_ms)
     reqs = []
     torch.cuda.synchronize()
     svc.t0 = time.perf_counter()
@@ -207,7 +230,7 @@ def run_trace(setups: dict, trace: list, slots: int, seed: int = 0, hbm_budget: 
         v = np.array([r.latency_ms for r in reqs if r.k == k])
         by_k[str(k)] = {"n": int(len(v)), "p50_ms": round(float(np.percentile(v, 50)), 2),
                         "p99_ms": round(float(np.percentile(v, 99)), 2)}
-    return {"slots": slots, "requests": len(reqs), "makespan_s": round(end, 3),
+    return {"slots": slots, "yield_ms": yield_ms, "requests": len(reqs), "makespan_s": round(end, 3),
             "proofs_per_s": round(len(reqs) / max(end, trace[-1][0] if trace else 1e-9), 3),
             "p50_ms": round(float(np.percentile(lat, 50)), 2) if len(lat) else None,
             "p99_ms": round(float(np.percentile(lat, 99)), 2) if len(lat) else None,

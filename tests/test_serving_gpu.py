@@ -32,18 +32,19 @@ def test_same_result_alone_and_in_mix(lib):
         alone[i] = r.result["proof"]
     # the same requests together: same seeds as above (run_trace derives them from the
     # index, so the mixed run re-creates them by index through a per-request seed base)
-    svc = serving.ProvingService(setups, slots=4)
-    reqs = [svc.submit(serving.Request(k, witness_seed=1000 + 7 * i + 1, blind_seed=1000 + 7 * i + 3, arrival=0.0))
-            for i, (_, k) in enumerate(mix)]
-    for r in reqs:
-        r.event.wait()
-    svc.close()
-    assert svc.max_in_flight > 1, "the mix never overlapped"
-    for i, r in enumerate(reqs):
-        assert r.error is None, r.error
-        assert r.result["proof"] == alone[i], f"request {i} (k={r.k}) differs in the mix"
-        s = setups[r.k]
-        assert pv.verify_proof(s.pk.vk, r.result["instance"], r.result["proof"], tau=s.params.tau), i
+    for yield_ms in (0.0, 20.0):  # plain, and larger proofs pausing between phases for smaller ones
+        svc = serving.ProvingService(setups, slots=4, yield_ms=yield_ms)
+        reqs = [svc.submit(serving.Request(k, witness_seed=1000 + 7 * i + 1, blind_seed=1000 + 7 * i + 3,
+                                           arrival=0.0)) for i, (_, k) in enumerate(mix)]
+        for r in reqs:
+            r.event.wait()
+        svc.close()
+        assert svc.max_in_flight > 1, "the mix never overlapped"
+        for i, r in enumerate(reqs):
+            assert r.error is None, r.error
+            assert r.result["proof"] == alone[i], f"request {i} (k={r.k}) differs in the mix (yield {yield_ms})"
+            s = setups[r.k]
+            assert pv.verify_proof(s.pk.vk, r.result["instance"], r.result["proof"], tau=s.params.tau), i
 
 
 def test_admission_respects_budget(lib):
