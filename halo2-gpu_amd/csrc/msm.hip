@@ -56,7 +56,8 @@ constexpr uint32_t R2_CHUNK = 8192; // entries per round-2 chunk (staged in LDS)
 constexpr uint32_t LIGHT = 32; // pieces of a split bucket summed by one thread
 constexpr uint32_t HEAVY_THREADS = 256;
 constexpr uint32_t HEAVY_BLOCKS = 512;
-constexpr uint32_t RED_L1 = 16; // buckets per first-level reduction segment (8 up to 2^16 buckets per window)
+constexpr uint32_t RED_L1 = 16; // at most this many buckets per first-level reduction segment
+constexpr uint64_t RED_THREADS = 1ull << 17; // first-level segments wanted: 2 waves per SIMD
 constexpr uint32_t TREE = 256; // threads (tree leaves) per work-group at later levels
 constexpr uint32_t TREE_K = 2; // pairs folded sequentially into each leaf
 constexpr uint32_t MAX_RED_LEVELS = 8;
@@ -193,9 +194,12 @@ MsmPlan plan_core(uint64_t n, uint32_t c, uint32_t J, uint64_t table_n, uint32_t
         s = 128;
     p.S = (uint32_t)s;
     p.nthr = (work + p.S - 1) / p.S;
-    // shorter first-level runs shorten the serial chain where the reduction's
-    // latency dominates (few buckets); with many buckets the extra tree work wins
-    const uint32_t l1 = p.B <= (1u << 16) ? RED_L1 / 2 : RED_L1;
+    // first-level segments: each is a serial chain of 2 L1 point additions, so L1 is
+    // the largest power of two in [2, RED_L1] that still leaves ~RED_THREADS chains
+    // to fill the chip; the tree levels take the rest
+    uint32_t l1 = 2;
+    while (l1 < RED_L1 && (uint64_t)p.Wb * p.B / (2 * l1) >= RED_THREADS)
+        l1 *= 2;
     p.L1 = p.B < l1 ? p.B : l1;
     p.Mlev[0] = p.B / p.L1;
     uint32_t lev = 0;
@@ -271,7 +275,7 @@ struct MsmBuffers {
     uint32_t* heavy; // nb queued split buckets + 1 counter
     PointSlot* part; // 2 * nthr partial sums of split buckets
     PointSlot* bsum; // nb
-    PointSlot* red[4]; // (T, S) ping-pong: W * Mlev[0] each
+    PointSlot* red[4]; // (T, S) ping-pong: Wb * Mlev[0] pairs in 0, 1; Wb * Mlev[1] in 2, 3
     void* scan_scratch;
     uint64_t* out; // 12 limbs per MSM
 };
@@ -284,7 +288,8 @@ void carve(WS& ws, const MsmPlan& p, MsmBuffers* b)
     uint64_t scan_n = nh1 + 1;
     if (nh2 + 1 > scan_n)
         scan_n = nh2 + 1;
-    const uint64_t nred = (uint64_t)p.Wb * p.Mlev[0];
+    const uint64_t nred = (uint64_t)p.Wb * p.Mlev[0]; // pairs after the first level
+    const uint64_t nred1 = p.nlev ? (uint64_t)p.Wb * p.Mlev[1] : 0; // the most any later level writes
     void* v[24];
     int k = 0;
     v[k++] = ws.take(p.table_n ? 0 : 64 * p.pts_count);
@@ -304,7 +309,7 @@ void carve(WS& ws, const MsmPlan& p, MsmBuffers* b)
     v[k++] = ws.take(128 * 2 * p.nthr);
     v[k++] = ws.take(128 * p.nb);
     for (int i = 0; i < 4; ++i)
-        v[k++] = ws.take(128 * nred);
+        v[k++] = ws.take(128 * (i < 2 ? nred : nred1));
     v[k++] = ws.take(scan_u32_scratch(scan_n));
     v[k++] = ws.take(128ull * p.Wb); // window sums exported to the host
     if (b) {
@@ -1060,14 +1065,15 @@ __global__ void __launch_bounds__(256) reduce_first_kernel(const PointSlot* __re
 // Tree level: work-group g of window w reduces the pairs
 // [g * cnt * K, (g + 1) * cnt * K) of the Mi pairs of that window
 // (answer = sum T + sum_m m S_m, S in units of the current segment size).
-// Leaf i folds K consecutive pairs with running sums (local weights 0..K-1,
-// S scaled by K); adjacent blocks of equal size then combine as
-// T = T_L + T_R + S_R, S = 2 (S_L + S_R); after log2(cnt) rounds the
-// block's (T, S) is in the same form with cnt * K times larger segments.
-// The rounds are a chain of dependent point operations on few lanes (latency,
-// not throughput, bounds them), so the two halves of a combine run on different
-// waves: threads [0, TREE) form T, threads [TREE, 2 TREE) form S of the same node,
-// which halves the chain (2 point operations per round instead of 4).
+// Leaf i folds K consecutive pairs (local weights 0..K-1, S scaled by K);
+// adjacent blocks of equal size then combine as T = T_L + T_R + S_R,
+// S = 2 (S_L + S_R); after log2(cnt) rounds the block's (T, S) is in the same
+// form with cnt * K times larger segments.
+// The rounds are a chain of dependent point operations, so the two halves of
+// every step run on different waves: threads [0, TREE) form T, threads
+// [TREE, 2 TREE) form S of the same node (2 point operations per round instead
+// of 4). Round d's nodes sit on the first cnt / 2d threads of each half, so the
+// waves with no node left skip the round instead of issuing it with idle lanes.
 __global__ void __launch_bounds__(2 * TREE) reduce_tree_kernel(const PointSlot* __restrict__ Ti,
     const PointSlot* __restrict__ Si, uint64_t Mi, uint32_t K, uint32_t log_k, PointSlot* __restrict__ To,
     PointSlot* __restrict__ So)
@@ -1082,36 +1088,41 @@ __global__ void __launch_bounds__(2 * TREE) reduce_tree_kernel(const PointSlot* 
     const bool s_role = threadIdx.x >= TREE; // whole waves: TREE is a multiple of 64
     const uint32_t i = s_role ? threadIdx.x - TREE : threadIdx.x;
     const bool keep_s = cnt > 1 || Mo > 1;
-    if (!s_role && i < cnt) {
-        G1X29 a = G1X29::inf(), run = G1X29::inf(), acc = G1X29::inf();
-        for (int j = (int)K - 1; j >= 0; --j) {
-            const uint64_t q = base + (uint64_t)i * K + (uint32_t)j;
-            a = G1X29::add(a, G1X29::load(Ti + q));
-            run = G1X29::add(run, G1X29::load(Si + q));
-            if (j > 0)
+    if (i < cnt) {
+        const uint64_t q0 = base + (uint64_t)i * K;
+        if (!s_role) { // T = sum_j T_j + sum_j j S_j
+            G1X29 a = G1X29::load(Ti + q0 + K - 1), run = G1X29::inf(), acc = G1X29::inf();
+            for (int j = (int)K - 1; j > 0; --j) {
+                run = G1X29::add(run, G1X29::load(Si + q0 + (uint32_t)j));
                 acc = G1X29::add(acc, run);
-        }
-        sT[i] = G1X29::add(a, acc);
-        if (keep_s)
+                a = G1X29::add(a, G1X29::load(Ti + q0 + (uint32_t)j - 1));
+            }
+            sT[i] = G1X29::add(a, acc);
+        } else if (keep_s) { // S = K sum_j S_j
+            G1X29 run = G1X29::load(Si + q0);
+            for (uint32_t j = 1; j < K; ++j)
+                run = G1X29::add(run, G1X29::load(Si + q0 + j));
             sS[i] = dbl_times(run, log_k);
+        }
     }
     for (uint32_t d = 1; d < cnt; d <<= 1) {
         __syncthreads();
-        G1X29 v;
+        const uint32_t l = i * 2 * d, r = l + d;
         const bool need_s = 2 * d < cnt || Mo > 1;
-        const bool act = (i & (2 * d - 1)) == 0 && i + d < cnt && (!s_role || need_s);
+        const bool act = r < cnt && (!s_role || need_s);
+        G1X29 v;
         if (act) {
             if (!s_role)
-                v = G1X29::add(G1X29::add(sT[i], sT[i + d]), sS[i + d]);
+                v = G1X29::add(G1X29::add(sT[l], sT[r]), sS[r]);
             else
-                v = G1X29::dbl(G1X29::add(sS[i], sS[i + d]));
+                v = G1X29::dbl(G1X29::add(sS[l], sS[r]));
         }
         __syncthreads();
         if (act) {
             if (!s_role)
-                sT[i] = v;
+                sT[l] = v;
             else
-                sS[i] = v;
+                sS[l] = v;
         }
     }
     __syncthreads();
