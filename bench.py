@@ -590,14 +590,16 @@ def main() -> int:
     if not args.no_autotune:
         from halo2_amd import autotune
 
-        tuned = autotune.tune(msm_logs=(16, 20, 22, 24), ntt_logs=(16, 18, 20, 22), apply=False)
+        tuned = autotune.tune(msm_logs=(16, 20, 22, 24), ntt_logs=(16, 18, 20, 22), table_logs=(16, 20, 22),
+                              apply=False)
         extra["autotune"] = {
-            "msm": {f"2^{lg}": {"default_c": v["default"], "best_c": v["best"],
-                                "default_ms": v["ms"][v["default"]], "best_ms": v["ms"][v["best"]]}
-                    for lg, v in tuned["msm"].items()},
+            key: {f"2^{lg}": {"default_c": v["default"], "best_c": v["best"],
+                              "default_ms": v["ms"][v["default"]], "best_ms": v["ms"][v["best"]]}
+                  for lg, v in tuned[key].items()} for key in ("msm", "msm_table")}
+        extra["autotune"].update({
             "ntt": {f"2^{lg}": {"best_tile_log2": v["best"], "default_ms": v["ms"][0],
                                 "best_ms": v["ms"][v["best"]]} for lg, v in tuned["ntt"].items()},
-        }
+        })
 
     # ------------------------------------------------------ CPU baseline
     progress("CPU baseline")

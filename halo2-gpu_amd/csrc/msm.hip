@@ -125,6 +125,8 @@ uint32_t default_width(uint64_t n, bool shared, uint32_t bits)
         c = 4;
     if (c > 20) // shared buckets too: measured 2^22 tables 7.6 ms at 20 bits vs 8.3 at 22, 2^24 equal
         c = 20;
+    if (!shared && c > 19) // plain 2^24: 26.4 ms at 19 bits (14 windows) vs 27.2 at 20 (13 windows)
+        c = 19;
     // same window count, narrowest width: the top window is as full as the others
     const uint32_t w0 = (bits + (uint32_t)c - 1) / (uint32_t)c;
     return (bits + w0 - 1) / w0;
@@ -160,6 +162,10 @@ uint32_t plain_msm_width(uint64_t npoints)
     // c = 16: 3.82 ms against 4.23 ms at c = 17; 2^16..2^18 are within noise of the rule)
     if (npoints >= (1ull << 19))
         return log2_floor(2 * npoints) - 5;
+    // 2^18 points: 1.30 ms at c = 16 against 1.54 at the rule's 15 (9 windows either way,
+    // but half as many entries per bucket to merge)
+    if (npoints >= (1ull << 18))
+        return 16;
     return default_width(2 * npoints, false, GLV_BITS);
 }
 

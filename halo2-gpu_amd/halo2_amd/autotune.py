@@ -73,6 +73,33 @@ def tune_msm_widths(logs=(16, 18, 20, 22, 24), spread: int = 2, reps: int = 3, s
     return out
 
 
+def tune_table_widths(logs=(16, 18, 20, 22), spread: int = 3, reps: int = 5, seed: int = 79) -> dict:
+    """For each 2^lg: build the fixed-base table at widths default - spread .. default and
+    time the precomputed MSM over all its points; returns {lg: {"default": c0, "best": c,
+    "ms": {c: ms}}}. A table serves only the width it was built with, so a chosen width
+    applies to tables built after it is set. The tables are left cleared."""
+    from . import msm
+
+    lib = _lib.load()
+    set_msm_widths({})
+    out = {}
+    for lg in logs:
+        n = 1 << lg
+        bases = msm.g1_mul_generator(msm.random_fr(n, seed), n)
+        scal = msm.random_fr(n, seed + lg)
+        c0 = int(lib.h2amd_msm_table_width(n))
+        times = {}
+        for c in range(max(4, c0 - spread), c0 + 1):
+            set_msm_widths({}, {lg: c})
+            table = msm.msm_precompute(bases, n)
+            times[c] = round(_median_ms(lambda: msm.multiexp_precomputed(scal, table, n, n), reps), 3)
+            del table
+        set_msm_widths({})
+        out[lg] = {"default": c0, "best": min(times, key=times.get), "ms": times}
+        del bases, scal
+    return out
+
+
 def tune_ntt_tiles(logs=(16, 18, 20, 22), candidates=(8, 9, 10, 11, 12), reps: int = 5, seed: int = 78) -> dict:
     """For each 2^lg: time the CosetFFT_Part transform (device in/out) with each tile;
     returns {lg: {"best": tile (0 = default), "ms": {tile: ms}}}. The table is left cleared."""
@@ -93,16 +120,19 @@ def tune_ntt_tiles(logs=(16, 18, 20, 22), candidates=(8, 9, 10, 11, 12), reps: i
     return out
 
 
-def tune(msm_logs=(16, 18, 20, 22, 24), ntt_logs=(16, 18, 20, 22), apply: bool = True) -> dict:
-    """Both tuners; with apply, the best choices become the library's tables."""
-    res = {"msm": tune_msm_widths(msm_logs), "ntt": tune_ntt_tiles(ntt_logs)}
+def tune(msm_logs=(16, 18, 20, 22, 24), ntt_logs=(16, 18, 20, 22), table_logs=(16, 18, 20, 22),
+         apply: bool = True) -> dict:
+    """All three tuners; with apply, the best choices become the library's tables."""
+    res = {"msm": tune_msm_widths(msm_logs), "ntt": tune_ntt_tiles(ntt_logs),
+           "msm_table": tune_table_widths(table_logs)}
     if apply:
         apply_tables(res)
     return res
 
 
 def apply_tables(res: dict) -> None:
-    set_msm_widths({int(lg): v["best"] for lg, v in res["msm"].items() if v["best"] != v["default"]})
+    set_msm_widths({int(lg): v["best"] for lg, v in res["msm"].items() if v["best"] != v["default"]},
+                   {int(lg): v["best"] for lg, v in res.get("msm_table", {}).items() if v["best"] != v["default"]})
     set_ntt_tiles({int(lg): v["best"] for lg, v in res["ntt"].items() if v["best"]})
 
 

@@ -67,6 +67,26 @@ def test_msm_widths_same_result(lib, co):
 
 
 @pytest.mark.gpu
+def test_table_widths_same_result(lib, co):
+    """A fixed-base table built at any width gives the plain MSM's result."""
+    from halo2_amd import device as dv, msm
+    from halo2_amd.autotune import set_msm_widths
+
+    n = 3000
+    ds = dv.to_device(rand_fr(co, n, 41))
+    dp = msm.g1_mul_generator(dv.to_device(rand_fr(co, n, 42)), n)
+    exp = msm.multiexp_device_scalars_device_bases(ds, dp, n)
+    try:
+        for w in (6, 9, 11, 12, 14):
+            set_msm_widths({}, {11: w})
+            assert int(lib.h2amd_msm_table_width(n)) == w
+            table = msm.msm_precompute(dp, n)
+            assert (msm.multiexp_precomputed(ds, table, n, n) == exp).all(), f"table width {w}"
+    finally:
+        set_msm_widths({})
+
+
+@pytest.mark.gpu
 def test_ntt_tiles_same_result(lib, co):
     from halo2_amd import device as dv, fields as F, ntt
     from halo2_amd.autotune import set_ntt_tiles
@@ -89,12 +109,15 @@ def test_ntt_tiles_same_result(lib, co):
 def test_tuner_round_trip(lib, co, tmp_path):
     from halo2_amd import autotune
 
-    res = autotune.tune(msm_logs=(12, 14), ntt_logs=(12, 14), apply=True)
+    res = autotune.tune(msm_logs=(12, 14), ntt_logs=(12, 14), table_logs=(12, 14), apply=True)
     try:
         for lg in (12, 14):
             m = res["msm"][lg]
             assert m["best"] in m["ms"] and m["default"] in m["ms"]
             assert int(lib.h2amd_msm_width(1 << lg)) == m["best"]
+            t = res["msm_table"][lg]
+            assert t["best"] in t["ms"] and t["default"] in t["ms"]
+            assert int(lib.h2amd_msm_table_width(1 << lg)) == t["best"]
             assert res["ntt"][lg]["best"] in res["ntt"][lg]["ms"]
         path = str(tmp_path / "tuned.json")
         autotune.save(res, path)
