@@ -87,6 +87,8 @@ def row_measurements(lib, dv, F, msm, torch, stream) -> dict:
     from halo2_amd.domain import EvaluationDomain
     from halo2_amd.quotient import Advice, Fixed, Product, Sum, custom_gates_evaluator, quotient_device
 
+    import numpy as np
+
     out = {}
 
     def rec(name, n, dt, bytes_per_elem):
@@ -131,7 +133,22 @@ def row_measurements(lib, dv, F, msm, torch, stream) -> dict:
     fixed = [msm.random_fr(n, seed=330 + i) for i in range(4)]
     rec("quotient_rules_standard_gate_2^22", n, _timed(
         lambda: quotient_device(ev, fixed, [], cols[:3], [], 5, n, 4), 5, torch), 32 * (7 + 1))
-    del perms, fixed, zmod, pscr
+    # lookup and permutation quotient identities over one extended part (2 sets of 2 columns)
+    qv = msm.random_fr(n, seed=335)
+    lcols = [msm.random_fr(n, seed=336 + i) for i in range(7)]  # table, z, a', s', l0, l_last, l_active
+    y = dv.to_device(F.mont_limbs(19).reshape(1, 4))
+    rec("quotient_lookups_2^22", n, _timed(lambda: _lib.check(lib._halo2_quotient_lookups(
+        dv.ptr(qv), *[dv.ptr(x) for x in lcols], dv.ptr(beta), dv.ptr(gamma), dv.ptr(y), n, stream)), 5, torch),
+        32 * (7 + 2))
+    zs2 = [lcols[1], lcols[2]]
+    ptab = [torch.tensor([dv.ptr(x) for x in group], dtype=torch.int64, device=dv.device())
+            for group in (zs2, perms, cols[:4])]
+    host_sc = [np.ascontiguousarray(F.mont_limbs(v).reshape(1, 4)) for v in (11, 13, 19, 5, 11, 7, F.omega(k))]
+    rec("quotient_permutation_2^22_2sets", n, _timed(lambda: _lib.check(lib._halo2_quotient_permutation(
+        dv.ptr(qv), *[dv.ptr(x) for x in lcols[4:7]], *[dv.ptr(t) for t in ptab],
+        *[ctypes.c_void_p(x.ctypes.data) for x in host_sc], 2, 2, 4, -1, 4, n, n, stream)), 5, torch),
+        32 * (2 + 4 + 4 + 3 + 2))
+    del perms, fixed, zmod, pscr, qv, lcols, zs2, ptab
     # coeff_to_extended 2^20 -> 2^22 (pad, zeta twist, NTT of the extended size)
     k = 20
     dom = EvaluationDomain(5, k)  # quotient degree 4: extended domain 2^22
@@ -140,8 +157,6 @@ def row_measurements(lib, dv, F, msm, torch, stream) -> dict:
         _timed(lambda: dom.coeff_to_extended_many_device([poly]), 5, torch), 160)
     del cols, poly
     # opening phase: evaluation at a point and division by (X - u) of a 2^22-coefficient polynomial
-    import numpy as np
-
     from halo2_amd.lookup import permute_expression_pair_device
 
     k = 22
