@@ -1092,7 +1092,10 @@ __global__ void __launch_bounds__(2 * TREE) reduce_tree_kernel(const PointSlot* 
     const uint64_t w = blockIdx.x / Mo, g = blockIdx.x % Mo;
     const uint64_t base = w * Mi + g * per_blk;
     const bool s_role = threadIdx.x >= TREE; // whole waves: TREE is a multiple of 64
-    const uint32_t i = s_role ? threadIdx.x - TREE : threadIdx.x;
+    // S node i on thread TREE + (i + 64) mod TREE: the waves of a work-group are dealt
+    // to the CU's 4 SIMDs in turn, so the late rounds' T wave (threads 0..63) and S
+    // wave (threads TREE + 64..) issue on different SIMDs instead of sharing one
+    const uint32_t i = s_role ? (threadIdx.x - TREE + TREE - 64) % TREE : threadIdx.x;
     const bool keep_s = cnt > 1 || Mo > 1;
     if (i < cnt) {
         const uint64_t q0 = base + (uint64_t)i * K;
