@@ -298,9 +298,10 @@ class VerifyingKey:
 @dataclass
 class ProvingKey:
     """keygen_pk for the circuit's fixed part (src/plonk/keygen.rs in halo2-axiom): the
-    domain, fixed polynomials, l0 / l_last / l_active and the permutation's sigma
-    columns (Lagrange and coefficients), all resident on the device, plus the
-    verifying key."""
+    domain, the fixed columns (coefficients, and their Lagrange values in
+    `fixed_values`), l0 / l_last / l_active and the permutation's sigma columns
+    (Lagrange and coefficients), all resident on the device, plus the verifying key.
+    pk_io writes and reads it as bytes."""
 
     domain: EvaluationDomain
     fixed_polys: list
@@ -314,6 +315,7 @@ class ProvingKey:
     lookup: object = None
     key_parts: dict = field(default_factory=dict)  # coset values of the fixed part, per extended part
     timings: dict = field(default_factory=dict)
+    fixed_values: list = field(default_factory=list)  # Lagrange
 
 
 def keygen(circuit: Circuit, params: msm.ParamsKZG | None = None) -> ProvingKey:
@@ -339,7 +341,8 @@ def keygen(circuit: Circuit, params: msm.ParamsKZG | None = None) -> ProvingKey:
     sigma_values = [polyops.generate_omegadelta_device(flat, dom.omega, DELTA, k, j * n, (j + 1) * n - 1, m)
                     for j in range(m)]
     sigma_polys = dom.lagrange_to_coeff_many_device_inputs(sigma_values)
-    pk = ProvingKey(dom, fixed_polys, l0, l_last, l_active, sigma_values, sigma_polys)
+    pk = ProvingKey(dom, fixed_polys, l0, l_last, l_active, sigma_values, sigma_polys,
+                    fixed_values=list(circuit.fixed))
     pk.gates = custom_gates_evaluator(list(cs.gates))
     (inputs, tables), = cs.lookups
     pk.lookup = lookup_evaluator(list(inputs), list(tables))
@@ -443,7 +446,7 @@ def create_proof(params: msm.ParamsKZG, pk: ProvingKey, circuit: Circuit, seed: 
     # 2. lookup commit_permuted (lookup/prover.rs:552-627): theta-compression of one
     #    column is the column itself; permuted pair, blinding rows, commitments
     theta = tr.squeeze()
-    inp, tab = circuit.advice[D], circuit.fixed[TABLE]
+    inp, tab = circuit.advice[D], (pk.fixed_values or circuit.fixed)[TABLE]
     p_in, p_tab = dv.empty_u64(4 * n), dv.empty_u64(4 * n)
     permute_expression_pair_device(inp, tab, p_in, p_tab, n, u)
     _set_rows(p_in, u, msm.random_fr(n - u, seed * 7 + 1))
