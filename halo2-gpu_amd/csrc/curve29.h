@@ -6,10 +6,11 @@
 // bases into it once per call and converts the result back.
 //
 // State invariant: the four coordinates of a G1X29 are normalised with values
-// < 2m (X and Y are brought below 2m by reduce_2m at the end of a formula,
-// ZZ and ZZZ are products). Affine inputs are canonical (< m). With inputs
-// below 2m every product below is < 1.06m, so each difference fits the 2m / 4m
-// offsets of sub2 / sub4; the comments give the bounds (in units of m).
+// < 2m (X is brought below 2m by reduce_2m at the end of a formula; Y is a sum of
+// two products under one reduction, mul_sum, < 1.2m; ZZ and ZZZ are products).
+// Affine inputs are canonical (< m). With inputs below 2m every product below is
+// < 1.06m, so each difference fits the 2m / 4m offsets of sub2 / sub4; the
+// comments give the bounds (in units of m).
 #pragma once
 
 #include "curve.h"
@@ -20,6 +21,8 @@ namespace h2 {
 using Fq29 = F29<Fq29P>;
 
 __device__ __forceinline__ Fq29 nsub2(const Fq29& a, const Fq29& b) { return Fq29::normalize(Fq29::sub2(a, b)); }
+// 4m - b, normalised (b normalised, < 4m): the negation mul_sum takes for a difference of products
+__device__ __forceinline__ Fq29 nneg4(const Fq29& b) { return Fq29::normalize(Fq29::sub4(Fq29::zero(), b)); }
 
 struct G1A29 {
     Fq29 x, y;
@@ -77,8 +80,8 @@ struct G1X29 {
         const Fq29 S2 = Fq29::normalize(Fq29::add(S, S)); // < 2.2
         G1X29 r;
         r.X = Fq29::reduce_2m(Fq29::normalize(Fq29::sub4(Fq29::sqr(M), S2)));
-        const Fq29 t = Fq29::mul(M, nsub2(S, r.X));
-        r.Y = Fq29::reduce_2m(nsub2(t, Fq29::mul(W, a.y)));
+        // M (S - X3) - W y with one reduction: < m + (3.3 * 3.1 + 1.1 * 4) m^2 / 2^261 < 1.2m
+        r.Y = Fq29::mul_sum(M, nsub2(S, r.X), W, nneg4(a.y));
         r.ZZ = V;
         r.ZZZ = W;
         return r;
@@ -100,8 +103,7 @@ struct G1X29 {
         const Fq29 S2 = Fq29::normalize(Fq29::add(S, S));
         G1X29 r;
         r.X = Fq29::reduce_2m(Fq29::normalize(Fq29::sub4(Fq29::sqr(M), S2)));
-        const Fq29 t = Fq29::mul(M, nsub2(S, r.X));
-        r.Y = Fq29::reduce_2m(nsub2(t, Fq29::mul(W, p.Y)));
+        r.Y = Fq29::mul_sum(M, nsub2(S, r.X), W, nneg4(p.Y)); // < 1.2m, as in dbl_affine
         r.ZZ = Fq29::mul(V, p.ZZ);
         r.ZZZ = Fq29::mul(W, p.ZZZ);
         return r;
@@ -129,8 +131,8 @@ struct G1X29 {
         const Fq29 T = Fq29::normalize(Fq29::add(PPP, Fq29::add(Q, Q))); // < 3.2
         G1X29 r;
         r.X = Fq29::reduce_2m(Fq29::normalize(Fq29::sub4(Fq29::sqr(R), T)));
-        const Fq29 t = Fq29::mul(R, nsub2(Q, r.X));
-        r.Y = Fq29::reduce_2m(nsub2(t, Fq29::mul(p.Y, PPP)));
+        // R (Q - X3) - Y1 PPP with one reduction: < m + (3.1 * 3.1 + 2 * 4) m^2 / 2^261 < 1.2m
+        r.Y = Fq29::mul_sum(R, nsub2(Q, r.X), p.Y, nneg4(PPP));
         r.ZZ = Fq29::mul(p.ZZ, PP);
         r.ZZZ = Fq29::mul(p.ZZZ, PPP);
         return r;
@@ -160,8 +162,7 @@ struct G1X29 {
         const Fq29 T = Fq29::normalize(Fq29::add(PPP, Fq29::add(Q, Q)));
         G1X29 r;
         r.X = Fq29::reduce_2m(Fq29::normalize(Fq29::sub4(Fq29::sqr(R), T)));
-        const Fq29 t = Fq29::mul(R, nsub2(Q, r.X));
-        r.Y = Fq29::reduce_2m(nsub2(t, Fq29::mul(S1, PPP)));
+        r.Y = Fq29::mul_sum(R, nsub2(Q, r.X), S1, nneg4(PPP)); // < 1.2m, as in add_affine
         r.ZZ = Fq29::mul(Fq29::mul(p.ZZ, q.ZZ), PP);
         r.ZZZ = Fq29::mul(Fq29::mul(p.ZZZ, q.ZZZ), PPP);
         return r;

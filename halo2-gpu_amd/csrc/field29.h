@@ -213,6 +213,43 @@ struct F29 {
         return r;
     }
 
+    // (a b + c d) / 2^261 with one reduction for both products: every operand
+    // normalised (limbs < 2^29, values < 2^259), so a column holds at most 27 products
+    // of < 2^58 (< 2^63). Returns normalised limbs of a value < m + (a b + c d) / 2^261.
+    __device__ __forceinline__ static F29 mul_sum(const F29& a, const F29& b, const F29& c, const F29& d)
+    {
+        uint32_t m[9];
+        uint64_t acc = 0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+#pragma unroll
+            for (int i = 0; i <= k; ++i) {
+                acc += (uint64_t)a.l[i] * b.l[k - i];
+                acc += (uint64_t)c.l[i] * d.l[k - i];
+            }
+#pragma unroll
+            for (int i = 0; i < k; ++i)
+                acc += (uint64_t)m[i] * P::M[k - i];
+            m[k] = ((uint32_t)acc * P::PINV) & MASK;
+            acc += (uint64_t)m[k] * P::M[0];
+            acc >>= 29;
+        }
+        F29 r;
+#pragma unroll
+        for (int k = 9; k < 17; ++k) {
+#pragma unroll
+            for (int i = k - 8; i <= 8; ++i) {
+                acc += (uint64_t)a.l[i] * b.l[k - i];
+                acc += (uint64_t)c.l[i] * d.l[k - i];
+                acc += (uint64_t)m[i] * P::M[k - i];
+            }
+            r.l[k - 9] = (uint32_t)acc & MASK;
+            acc >>= 29;
+        }
+        r.l[8] = (uint32_t)acc;
+        return r;
+    }
+
     // mul(a, a) with the 36 cross products taken once against doubled limbs
     // (45 instead of 81 multiply-adds for the product half). Same operand
     // limits as mul with both operands a (limbs < 2^30), same result: every
