@@ -54,9 +54,10 @@ def parse():
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the Poisson-arrival serving run (0 skips it)")
     ap.add_argument("--serve-rates", default="4,28", help="arrival rates (per second) of the serving runs")
-    ap.add_argument("--serve-modes", default="fifo,continuous,continuous_yield",
+    ap.add_argument("--serve-modes", default="fifo,continuous,continuous_yield,continuous_sized",
                     help="serving modes to run (continuous_yield: larger proofs pause between phases "
-                         "while a smaller one runs)")
+                         "while a smaller one runs; continuous_sized: that, and proofs of the largest k "
+                         "one at a time)")
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the serving measurement (e.g. --serve-seconds 600 --serve-rates 4)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -279,12 +280,14 @@ def proof_measurements(msm, torch, k: int, reps: int) -> dict:
 
 def serving_measurements(torch, seconds: float, rates=(4.0, 28.0), ks=(16, 18, 20, 22),
                          weights=(0.4, 0.3, 0.2, 0.1), seed: int = 55, slots: int = 4,
-                         modes=("fifo", "continuous", "continuous_yield")) -> dict:
+                         modes=("fifo", "continuous", "continuous_yield", "continuous_sized")) -> dict:
     """The proving service (halo2_amd.serving) on open-loop Poisson traces: requests for
     the standard-gate circuit at k drawn with `weights`, each with a fresh witness, every
-    k's SRS, fixed-base tables and keys resident. Each trace is served twice: one proof
-    at a time (FIFO, the plain GPU path) and with continuous batching (`slots` proofs in
-    flight, admitted by their measured HBM footprint). The first rate is the config's
+    k's SRS, fixed-base tables and keys resident. Each trace is served in each mode: one
+    proof at a time (FIFO, the plain GPU path); continuous batching (`slots` proofs in
+    flight, admitted by their measured HBM footprint); that with larger proofs pausing
+    between phases while a smaller one runs (yield); and that with the largest k's
+    proofs one at a time (sized). The first rate is the config's
     4/s (the device mostly idle); the second is just above the one-at-a-time capacity
     (~25/s for this mix: a mean service time of ~40 ms), so FIFO falls behind and its
     queue grows while the batched service, whose capacity is higher, keeps up."""
@@ -296,12 +299,14 @@ def serving_measurements(torch, seconds: float, rates=(4.0, 28.0), ks=(16, 18, 2
     for rate in rates:
         trace = serving.poisson_trace(seconds, rate, ks, weights, seed + int(rate * 10))
         runs = {}
-        for name, nslots, yield_ms in (("fifo", 1, 0.0), ("continuous", slots, 0.0),
-                                       ("continuous_yield", slots, 40.0)):
+        for name, nslots, yield_ms, excl in (("fifo", 1, 0.0, None), ("continuous", slots, 0.0, None),
+                                             ("continuous_yield", slots, 40.0, None),
+                                             ("continuous_sized", slots, 40.0, max(ks))):
             if name not in modes:
                 continue
             progress(f"serving rate {rate:g}/s {name}: {len(trace)} requests")
-            r = serving.run_trace(setups, trace, nslots, seed=seed * 1000, progress=progress, yield_ms=yield_ms)
+            r = serving.run_trace(setups, trace, nslots, seed=seed * 1000, progress=progress, yield_ms=yield_ms,
+                                  exclusive_k=excl)
             r.pop("results")
             runs[name] = r
         out["runs"][f"rate_{rate:g}"] = runs

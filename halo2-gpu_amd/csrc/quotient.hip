@@ -92,6 +92,7 @@ __global__ void __launch_bounds__(kBlock) quotient_permutation_kernel(Fr* __rest
         pw[0] = Fr::one();
         pw[1] = omega;
         s_base = cur_ext_omega * Fr::pow_u64(omega, (uint64_t)blockIdx.x * blockDim.x);
+    } else if (threadIdx.x == 64) { // the other exponentiation on the second wave, beside it
         s_step = Fr::pow_u64(omega, stride);
     }
     __syncthreads();

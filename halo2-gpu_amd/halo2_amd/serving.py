@@ -84,13 +84,19 @@ class ProvingService:
     proof of a smaller circuit is running and no request is waiting, so a short proof
     that arrives during a long one is not stretched by sharing the device with it. The
     pause applies only while the service keeps up: under a backlog every proof runs
-    flat out, since pausing then only delays the queue. 0 disables."""
+    flat out, since pausing then only delays the queue. 0 disables.
+
+    `exclusive_k`: proofs of circuits at or above this size run one at a time (a later
+    one waits while smaller requests behind it are admitted): one such proof's launches
+    already fill the device, and two side by side stretch each other to twice their
+    latency. None disables."""
 
     def __init__(self, setups: dict, slots: int = 4, hbm_budget: int | None = None, reserve: int = 4 << 30,
-                 yield_ms: float = 0.0):
+                 yield_ms: float = 0.0, exclusive_k: int | None = None):
         self.setups = setups
         self.slots = slots
         self.yield_s = yield_ms / 1e3
+        self.exclusive_k = exclusive_k
         if hbm_budget is None:
             total = torch.cuda.get_device_properties(torch.cuda.current_device()).total_memory
             hbm_budget = total - torch.cuda.memory_allocated() - reserve
@@ -129,8 +135,12 @@ class ProvingService:
             w.join()
 
     def _take(self) -> Request | None:
-        """Under the lock: the first waiting request (arrival order) whose footprint fits."""
+        """Under the lock: the first waiting request (arrival order) whose footprint fits
+        (and, with exclusive_k, that is not a second large proof)."""
+        big_running = self.exclusive_k is not None and any(k >= self.exclusive_k for k in self._running_k.values())
         for i, req in enumerate(self.waiting):
+            if big_running and req.k >= self.exclusive_k:
+                continue
             need = self.setups[req.k].footprint
             if self.in_flight_bytes + need <= self.budget:
                 del self.waiting[i]
@@ -195,13 +205,14 @@ def poisson_trace(seconds: float, rate: float, ks, weights, seed: int) -> list:
 
 
 def run_trace(setups: dict, trace: list, slots: int, seed: int = 0, hbm_budget: int | None = None,
-              progress=None, progress_every_s: float = 30.0, yield_ms: float = 0.0) -> dict:
+              progress=None, progress_every_s: float = 30.0, yield_ms: float = 0.0,
+              exclusive_k: int | None = None) -> dict:
     """Open-loop run: every request is submitted at its arrival time whatever the
     service's state; each gets a fresh witness (seed + index) and blinders. Returns
     throughput, latency percentiles (arrival to finished proof), per-k latencies and the
     peak number of proofs in flight. `progress(message)`, when given, is called about
     every `progress_every_s` seconds of the run."""
-    svc = ProvingService(setups, slots=slots, hbm_budget=hbm_budget, yield_ms=yield_ms)
+    svc = ProvingService(setups, slots=slots, hbm_budget=hbm_budget, yield_ms=yield_ms, exclusive_k=exclusive_k)
     reqs = []
     torch.cuda.synchronize()
     svc.t0 = time.perf_counter()
@@ -230,7 +241,8 @@ def run_trace(setups: dict, trace: list, slots: int, seed: int = 0, hbm_budget: 
         v = np.array([r.latency_ms for r in reqs if r.k == k])
         by_k[str(k)] = {"n": int(len(v)), "p50_ms": round(float(np.percentile(v, 50)), 2),
                         "p99_ms": round(float(np.percentile(v, 99)), 2)}
-    return {"slots": slots, "yield_ms": yield_ms, "requests": len(reqs), "makespan_s": round(end, 3),
+    return {"slots": slots, "yield_ms": yield_ms, "exclusive_k": exclusive_k, "requests": len(reqs),
+            "makespan_s": round(end, 3),
             "proofs_per_s": round(len(reqs) / max(end, trace[-1][0] if trace else 1e-9), 3),
             "p50_ms": round(float(np.percentile(lat, 50)), 2) if len(lat) else None,
             "p99_ms": round(float(np.percentile(lat, 99)), 2) if len(lat) else None,

@@ -1,8 +1,10 @@
 """The continuous-batching proving service (halo2_amd.serving): every result is the
 same alone and in a mixed batch (the same witness and blinding seeds give the same
 proof bytes whether the proof runs by itself or beside proofs of other sizes on other
-streams), every proof verifies, and admission respects the HBM budget (a budget for one
-proof serialises the service; a budget for two never has more than two in flight)."""
+streams), also with larger proofs yielding to smaller ones and with the largest size
+run one at a time, every proof verifies, and admission respects the HBM budget (a
+budget for one proof serialises the service; a budget for two never has more than two
+in flight)."""
 import pytest
 
 pytestmark = pytest.mark.gpu
@@ -45,6 +47,22 @@ def test_same_result_alone_and_in_mix(lib):
             assert r.result["proof"] == alone[i], f"request {i} (k={r.k}) differs in the mix (yield {yield_ms})"
             s = setups[r.k]
             assert pv.verify_proof(s.pk.vk, r.result["instance"], r.result["proof"], tau=s.params.tau), i
+
+
+def test_large_proofs_one_at_a_time(lib):
+    """exclusive_k: proofs at or above it never overlap; smaller ones still run beside
+    them, and every proof is the one it would be alone."""
+    from halo2_amd import serving
+
+    setups = _setups((10, 11))
+    trace = [(0.0, 11), (0.0, 11), (0.0, 10), (0.0, 11), (0.0, 10), (0.0, 10)]
+    free = serving.run_trace(setups, trace, slots=4, seed=9)
+    one = serving.run_trace(setups, trace, slots=4, seed=9, yield_ms=20.0, exclusive_k=11)
+    big = sorted((r for r in one["results"] if r.k == 11), key=lambda r: r.start)
+    for a, b in zip(big, big[1:]):
+        assert b.start >= a.done, "two k = 11 proofs ran at once"
+    assert one["max_in_flight"] >= 2 and one["all_checks_hold"]
+    assert [r.result["proof"] for r in one["results"]] == [r.result["proof"] for r in free["results"]]
 
 
 def test_admission_respects_budget(lib):
