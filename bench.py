@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--proof-k", type=int, default=20, help="k of the standard-gate proof measured under extra")
     ap.add_argument("--proof-reps", type=int, default=10)
     ap.add_argument("--no-proof", action="store_true", help="skip the end-to-end proof measurement")
+    ap.add_argument("--proof-phase-gap-ms", type=float, default=0.0,
+                    help="profiling aid: one extra, uncounted proof last that pauses this long after each phase")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the Poisson-arrival serving run (0 skips it)")
     ap.add_argument("--serve-rates", default="4,28", help="arrival rates (per second) of the serving runs")
@@ -243,12 +245,15 @@ def batch_measurements(msm, torch, seed: int = 2024, count: int = 32) -> dict:
     return out
 
 
-def proof_measurements(msm, torch, k: int, reps: int) -> dict:
+def proof_measurements(msm, torch, k: int, reps: int, phase_gap_ms: float = 0.0) -> dict:
     """End-to-end proofs of the standard-gate circuit at 2^k rows (halo2_amd.prover:
     5 advice columns, copy cycles, a 16-bit range lookup, SHPLONK opening), one after
     another on one GPU with the proving key, the SRS and its fixed-base tables resident
     (built by the untimed warm-up proof): proofs/s, p50/p99 latency
-    and the per-phase split of the median proof. Every proof's self-checks must hold."""
+    and the per-phase split of the median proof. Every proof's self-checks must hold.
+    With `phase_gap_ms`, one more proof (not counted) runs last, 0.3 s after the
+    others, pausing that long after each phase, so that a kernel trace shows its phases
+    apart."""
     import numpy as np
 
     from halo2_amd import prover
@@ -275,6 +280,11 @@ def proof_measurements(msm, torch, k: int, reps: int) -> dict:
     for streams in (2, 4):
         out[f"in_flight_{streams}"] = proofs_in_flight(prover, params, pk, circuit, torch, streams,
                                                        max(2, reps // 2))
+    if phase_gap_ms > 0:
+        torch.cuda.synchronize()
+        time.sleep(0.3)
+        prover.create_proof(params, pk, circuit, seed=reps + 1, phase_hook=lambda _: time.sleep(phase_gap_ms / 1e3))
+        torch.cuda.synchronize()
     return out
 
 
@@ -597,7 +607,8 @@ def main() -> int:
     # ------------------------- end-to-end proof of the standard-gate circuit
     progress("proofs")
     if not args.no_proof:
-        extra[f"proof_k{args.proof_k}"] = proof_measurements(msm, torch, args.proof_k, args.proof_reps)
+        extra[f"proof_k{args.proof_k}"] = proof_measurements(msm, torch, args.proof_k, args.proof_reps,
+                                                             args.proof_phase_gap_ms)
 
     # ---------------- proving service: Poisson arrivals over k in {16..22}
     progress("serving")
