@@ -230,7 +230,21 @@ struct PassParams {
     uint32_t scale[9]; // iFFT scale in the 2^261 scale
 };
 
-// LDS holds the tile as 9 limb planes: limb q of slot i at lds[q * RG + i]
+// Slot i of the tile lives at word lds_slot(i) of each plane: the low 6 bits of i
+// XOR a function of the higher bits, a permutation inside every 64-slot block (so a
+// range of whole blocks keeps its words). In slot order the load's bit-reversed
+// writes put all 64 lanes of a wave on one bank (64-way conflicts), the first stage
+// pairs' strided accesses 4-8 lanes per bank, and the multi-column store up to 16;
+// through this permutation every one of these patterns, for every tile shape the
+// plans use, spreads a wave over distinct banks (64 banks; 2 lanes per bank with 32),
+// as enumerating the patterns shows.
+__device__ __forceinline__ uint32_t lds_slot(uint32_t i)
+{
+    const uint32_t x = i >> 6;
+    return i ^ (((x ^ (x >> 6)) & 63u) ^ ((x & 3u) * 20u));
+}
+
+// LDS holds the tile as 9 limb planes: limb q of (physical) word i at lds[q * RG + i]
 __device__ __forceinline__ Fr29 lds_get(const uint32_t* lds, uint32_t RG, uint32_t i)
 {
     Fr29 r;
@@ -291,7 +305,7 @@ __device__ __forceinline__ void ntt_pass_tile(
             x = Fr29::zero();
         }
         const uint32_t slot = (g << log_r) + (__brev(n1) >> (32 - log_r));
-        lds_put(lds, RG, slot, x);
+        lds_put(lds, RG, lds_slot(slot), x);
     }
 
     // Stages go in pairs (radix 4: four elements per thread through LDS once
@@ -303,10 +317,11 @@ __device__ __forceinline__ void ntt_pass_tile(
         __syncthreads();
         for (uint32_t q = threadIdx.x; q < (RG >> 1); q += T) {
             const uint32_t i0 = q << 1;
-            const Fr29 a = lds_get(lds, RG, i0);
-            const Fr29 c = lds_get(lds, RG, i0 + 1);
-            lds_put(lds, RG, i0, Fr29::add(a, c));
-            lds_put(lds, RG, i0 + 1, Fr29::sub2(a, c));
+            const uint32_t p0 = lds_slot(i0), p1 = lds_slot(i0 + 1);
+            const Fr29 a = lds_get(lds, RG, p0);
+            const Fr29 c = lds_get(lds, RG, p1);
+            lds_put(lds, RG, p0, Fr29::add(a, c));
+            lds_put(lds, RG, p1, Fr29::sub2(a, c));
         }
         s = 1;
     }
@@ -315,8 +330,9 @@ __device__ __forceinline__ void ntt_pass_tile(
     // its blocks (4h slots) fit in 256 slots or a column has at most 256 slots
     // (q -> slot ranges are a fixed function of q for every such pair): those
     // pairs need only wave-level ordering of the LDS accesses (LDS serves one
-    // wave's instructions in order), not a work-group barrier. The first pair
-    // after the load or the radix-2 stage reads other waves' slots.
+    // wave's instructions in order), not a work-group barrier (lds_slot keeps a
+    // 256-slot range in the same words). The first pair after the load or the
+    // radix-2 stage reads other waves' slots.
     const uint32_t quarterR = R >> 2;
     const uint32_t s_first = s;
     for (; s + 1 < log_r; s += 2) {
@@ -334,10 +350,12 @@ __device__ __forceinline__ void ntt_pass_tile(
             const uint32_t pos = qq & (h - 1);
             const uint32_t blk = qq >> s;
             const uint32_t i0 = (g << log_r) + (blk << (s + 2)) + pos;
-            Fr29 x0 = lds_get(lds, RG, i0);
-            Fr29 x1 = lds_get(lds, RG, i0 + h);
-            Fr29 x2 = lds_get(lds, RG, i0 + 2 * h);
-            Fr29 x3 = lds_get(lds, RG, i0 + 3 * h);
+            const uint32_t p0 = lds_slot(i0), p1 = lds_slot(i0 + h), p2 = lds_slot(i0 + 2 * h),
+                           p3 = lds_slot(i0 + 3 * h);
+            Fr29 x0 = lds_get(lds, RG, p0);
+            Fr29 x1 = lds_get(lds, RG, p1);
+            Fr29 x2 = lds_get(lds, RG, p2);
+            Fr29 x3 = lds_get(lds, RG, p3);
             if (s) {
                 const Fr29 w = Fr29::load32(pp.twr + (pos << (log_r - 1 - s)));
                 x0 = Fr29::normalize(x0);
@@ -351,10 +369,10 @@ __device__ __forceinline__ void ntt_pass_tile(
             y1 = Fr29::normalize(y1);
             y2 = Fr29::mul(y2, Fr29::load32(pp.twr + (pos << (log_r - 2 - s))));
             y3 = Fr29::mul(y3, Fr29::load32(pp.twr + ((pos + h) << (log_r - 2 - s))));
-            lds_put(lds, RG, i0, Fr29::add(y0, y2));
-            lds_put(lds, RG, i0 + 2 * h, Fr29::sub2(y0, y2));
-            lds_put(lds, RG, i0 + h, Fr29::add(y1, y3));
-            lds_put(lds, RG, i0 + 3 * h, Fr29::sub2(y1, y3));
+            lds_put(lds, RG, p0, Fr29::add(y0, y2));
+            lds_put(lds, RG, p2, Fr29::sub2(y0, y2));
+            lds_put(lds, RG, p1, Fr29::add(y1, y3));
+            lds_put(lds, RG, p3, Fr29::sub2(y1, y3));
         }
     }
     __syncthreads();
@@ -374,7 +392,7 @@ __device__ __forceinline__ void ntt_pass_tile(
             k1 = f & (R - 1);
             g = f >> log_r;
         }
-        Fr29 x = lds_get(lds, RG, (g << log_r) + k1);
+        Fr29 x = lds_get(lds, RG, lds_slot((g << log_r) + k1));
         const uint32_t j = j0 + g;
         const uint32_t K = j & lmask;
         const uint32_t n2 = j >> log_l;
