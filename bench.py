@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
 # HBM bytes per ntt_pass launch at 2^24 measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # scaled by the 1 GiB copy calibration (FETCH x2.0, WRITE x1.0); profiles/ntt_pass_coset_2_24.md.
-NTT_PASS_TRAFFIC_2_24 = 1.108e9
+NTT_PASS_TRAFFIC_2_24 = 1.101e9
 
 
 def parse():
