@@ -34,7 +34,7 @@
 // form, the layout of the reference's `cukernel_to_affine` output.
 #include <atomic>
 #include <mutex>
-#include <memory>
+#include <optional>
 #include <vector>
 
 #include "common.h"
@@ -1361,13 +1361,23 @@ void host_finalize(const uint64_t* win, uint32_t Wj, uint32_t c, uint32_t J, uin
 
 // Copies the J x Wj window sums of a finished pipeline to the host, waits, and
 // writes the J 96-byte results to `h_out`.
+// A single MSM's window sums (at most 64 windows: c >= 4) land in a stack buffer, so the
+// single-MSM entry points, the scratch-free one included, allocate nothing on the host
+// either; only a batch's J x W sums take a heap buffer.
 RustError finish_on_host(const MsmPlan& p, const MsmBuffers& b, void* h_out, hipStream_t stream)
 {
     const uint32_t Wj = p.Wb / p.J;
-    std::vector<uint64_t> win(16ull * p.Wb);
-    H2_TRY(hipMemcpyAsync(win.data(), b.out, 128ull * p.Wb, hipMemcpyDeviceToHost, stream));
+    constexpr uint32_t kStackWindows = 64;
+    uint64_t local[16 * kStackWindows];
+    std::vector<uint64_t> heap;
+    uint64_t* win = local;
+    if (p.Wb > kStackWindows) {
+        heap.resize(16ull * p.Wb);
+        win = heap.data();
+    }
+    H2_TRY(hipMemcpyAsync(win, b.out, 128ull * p.Wb, hipMemcpyDeviceToHost, stream));
     H2_TRY(hipStreamSynchronize(stream));
-    host_finalize(win.data(), Wj, p.c, p.J, (uint64_t*)h_out);
+    host_finalize(win, Wj, p.c, p.J, (uint64_t*)h_out);
     return ok();
 }
 
@@ -1537,7 +1547,7 @@ RustError msm_device(const MsmPlan& p, const MsmBuffers& b, const Fr* d_scalars,
 {
     const unsigned T = 256;
     ProfScope prof_all("msm_total", stream);
-    std::unique_ptr<ProfScope> prof_sort(new ProfScope("msm_sort", stream));
+    std::optional<ProfScope> prof_sort(std::in_place, "msm_sort", stream); // ends before the buckets
     if (d_scalars && p.glv) {
         glv_split_kernel<<<blocks_for(p.nbase, T), T, 0, stream>>>(d_scalars, b.canon, p.nbase);
         H2_LAUNCH_CHECK();
