@@ -8,10 +8,9 @@
 // Evaluation: each work-group folds a span of coefficients (Horner in
 // u^256 per thread, coalesced reads), scales by the span's power of u and
 // sums; one small kernel adds the group sums. Division by (X - u):
-// q_j = a_{j+1} + u q_{j+1}, a linear recurrence solved in runs of 64 — each
-// run is solved locally with a zero boundary, the run heads obey the same
-// recurrence in u^64 (solved recursively), and a fix-up adds u^(e-j) q_e to
-// every output of a run ending at e.
+// q_j = a_{j+1} + u q_{j+1}, a linear recurrence solved as a suffix scan: 8
+// outputs per thread, a work-group scan over 2048-output tiles, one work-group
+// scan over the tiles, then every tile again with its carry.
 #include "common.h"
 #include "field.h"
 #include "field29.h"
@@ -249,142 +248,380 @@ unsigned ew_blocks(uint64_t n)
 }
 
 // --------------------------------------------------------- kate division
+//
+// q_j = b_j + u q_{j+1} for j < len, q_len = 0, b_j = src[j + 1], as a suffix scan
+// on three levels, every level on the 29-bit form (u and its powers in the 2^261
+// scale, data in the stored 2^256 scale):
+//  * outputs: tiles of kKateTile = 2048, one 256-thread work-group each; thread k
+//    owns the 8 consecutive outputs [8k, 8k + 8) of its tile, read through LDS (the
+//    tile is loaded with coalesced reads and transposed there);
+//  * threads (scan pass): a thread solves its outputs from its top with a zero
+//    boundary (7 dependent products) and a Kogge-Stone suffix scan over the
+//    work-group in LDS (8 steps, multiplier u^(8d) at distance d) gives S'_k, the value
+//    at thread k's bottom with a zero tile top; S'_k is kept for the output pass and
+//    S'_0 = A_t is the tile's;
+//  * tiles: the A_t are combined the same way by one work-group (kate_carries_kernel)
+//    into the true values C_t = q_(tT);
+//  * output pass: thread k's carry c_k = S'_(k+1) + u^(8 (255 - k)) C_(t+1) (one
+//    product, the power from a table), then its 8 outputs by the recurrence itself.
+// 7 + 8 + 1 + 8 products per thread of 8 outputs, at most 16 of them dependent in one
+// pass. Launches: powers of u (one work-group), the scan pass (plus one work-group
+// squaring out the carries' multipliers beside the tiles), the carries (more than one
+// tile only), the outputs.
 
-// outputs per thread in one recurrence level: each level is a chain of kRun
-// dependent products on one lane, so short runs (more threads, one more level at
-// 2^20) cut the latency that bounds the division
-constexpr uint64_t kRun = 32;
-constexpr int kMaxLevels = 13; // 32^13 > 2^64
+constexpr unsigned kKateT = 256; // threads per tile work-group
+constexpr unsigned kKateE = 8; // outputs per thread
+constexpr uint64_t kKateTile = (uint64_t)kKateT * kKateE;
+constexpr unsigned kKateScanSteps = 8; // log2 kKateT
+constexpr unsigned kKateP8 = kKateT + 1; // u^(8 j), j = 0..kKateT
+constexpr unsigned kKateP8Stride = 264;
 
-struct KateLevel {
-    uint64_t len; // outputs at this level
-    uint64_t runs; // ceil(len / kRun)
+// powers of u in the 2^261 scale, normalised 29-bit limbs below 2m
+struct KatePw {
+    uint32_t u[16];
+    uint32_t carry[kKateScanSteps][16]; // u^(kKateTile g 2^s), g = 2^log_g tiles per carries thread
+    uint32_t p8[9][kKateP8Stride]; // limb w of u^(8 j) at p8[w][j]
 };
 
-// levels of the recursion for `outputs` outputs (>= 1)
-int kate_levels(uint64_t outputs, KateLevel* lv)
+// the thread's scanned values of one tile, limb planes: S'_k limb w at [w * kKateT + k]
+constexpr uint64_t kKateScanWords = 9ull * kKateT;
+
+__device__ __forceinline__ Fr29 pw_u(const KatePw* pw)
 {
-    int k = 0;
-    uint64_t len = outputs;
-    while (true) {
-        lv[k].len = len;
-        lv[k].runs = (len + kRun - 1) / kRun;
-        ++k;
-        if (lv[k - 1].runs <= 1 || k == kMaxLevels)
-            break;
-        len = lv[k - 1].runs;
-    }
-    return k;
+    Fr29 r;
+#pragma unroll
+    for (int w = 0; w < 9; ++w)
+        r.l[w] = pw->u[w];
+    return r;
 }
 
-template <class WS>
-void kate_carve(WS& ws, const KateLevel* lv, int levels, Fr** pw, Fr** heads, Fr** solved)
+__device__ __forceinline__ Fr29 pw_p8(const KatePw* pw, unsigned j)
 {
-    for (int i = 0; i < levels; ++i) {
-        pw[i] = (Fr*)ws.take(32 * (kRun + 1));
-        heads[i] = (Fr*)ws.take(32 * (lv[i].runs + 1));
-        solved[i] = (Fr*)ws.take(32 * (lv[i].runs + 1));
+    Fr29 r;
+#pragma unroll
+    for (int w = 0; w < 9; ++w)
+        r.l[w] = pw->p8[w][j];
+    return r;
+}
+
+// the same value held per lane: products of uniform operands would otherwise go to the
+// scalar unit, which runs a 29-bit product about three times slower than one wave
+__device__ __forceinline__ Fr29 per_lane(Fr29 x)
+{
+#pragma unroll
+    for (int w = 0; w < 9; ++w)
+        asm volatile("" : "+v"(x.l[w]));
+    return x;
+}
+
+// LDS slot of tile element e: a permutation inside each 64-slot block, so that both
+// the coalesced pattern (64 consecutive e per wave) and the per-thread pattern
+// (e = 8 lane + i) put the 64 lanes of a wave on distinct banks
+__device__ __forceinline__ unsigned kate_slot(unsigned e) { return e ^ ((e >> 6) & 7u); }
+
+// x + y mult, brought below 2m: x, y, mult normalised below 2m (or x with limbs
+// < 2^30 below 2.1m)
+__device__ __forceinline__ Fr29 kate_step(const Fr29& x, const Fr29& y, const Fr29& mult)
+{
+    return Fr29::reduce_2m(Fr29::normalize(Fr29::add(x, Fr29::mul(y, mult))));
+}
+
+// Kogge-Stone suffix scan over the work-group's kKateT threads with a zero top: with
+// X_k = x of thread k, returns S_k = sum_(k <= i < kKateT) X_i U^(i - k),
+// U^(2^s) = mult(s). x normalised below 2m; returns normalised below 2m.
+// lds: 2 * 9 * kKateT words.
+template <class Mult>
+__device__ __forceinline__ Fr29 kate_block_scan(Fr29 x, uint32_t* lds, Mult mult)
+{
+    const unsigned k = threadIdx.x;
+#pragma unroll 1
+    for (unsigned s = 0; s < kKateScanSteps; ++s) {
+        const unsigned d = 1u << s;
+        // double buffer: the step after next rewrites this buffer only after the next
+        // barrier, which every thread reaches after its read below
+        uint32_t* buf = lds + (s & 1) * 9 * kKateT;
+#pragma unroll
+        for (int w = 0; w < 9; ++w)
+            buf[w * kKateT + k] = x.l[w];
+        __syncthreads();
+        if (k + d < kKateT) {
+            Fr29 y;
+#pragma unroll
+            for (int w = 0; w < 9; ++w)
+                y.l[w] = buf[w * kKateT + k + d];
+            x = kate_step(x, y, mult(s));
+        }
     }
+    return x;
+}
+
+// One work-group of kKateT threads: u, and u^(8 j) for j <= kKateT by doubling in LDS
+// (u^8 by 3 squarings, then 8 rounds in which threads i < h set
+// p[h + 1 + i] = p[1 + i] p[h]): 11 dependent products.
+__global__ void __launch_bounds__(kKateT) kate_table_kernel(const Fr* __restrict__ d_u, KatePw* __restrict__ pw)
+{
+    __shared__ uint32_t p[9][kKateP8Stride];
+    const unsigned t = threadIdx.x;
+    // u in the 2^261 scale: its stored word shifted by 5 bits (< 32m), reduced below 2m
+    const Fr29 u = per_lane(Fr29::reduce_2m(Fr29::shl5(Fr29::load32(d_u))));
+    if (t == 0) {
+        const Fr29 u8 = Fr29::sqr(Fr29::sqr(Fr29::sqr(u)));
+        const Fr29 one = Fr29::one();
+#pragma unroll
+        for (int w = 0; w < 9; ++w) {
+            p[w][0] = one.l[w];
+            p[w][1] = u8.l[w];
+            pw->u[w] = u.l[w];
+        }
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (unsigned h = 1; h < kKateT; h <<= 1) {
+        if (t < h) {
+            Fr29 a, b;
+#pragma unroll
+            for (int w = 0; w < 9; ++w) {
+                a.l[w] = p[w][1 + t];
+                b.l[w] = p[w][h];
+            }
+            const Fr29 r = Fr29::mul(a, b);
+#pragma unroll
+            for (int w = 0; w < 9; ++w)
+                p[w][h + 1 + t] = r.l[w];
+        }
+        __syncthreads();
+    }
+    for (unsigned j = t; j < kKateP8; j += kKateT)
+#pragma unroll
+        for (int w = 0; w < 9; ++w)
+            pw->p8[w][j] = p[w][j];
+}
+
+// Tile passes. SCAN: S'_k of every thread into scan[t * kKateScanWords ...] and A_t into
+// agg[t]; block `tiles` (one past the last tile) squares out the carries' multipliers
+// u^(kKateTile 2^(log_g + s)) instead. !SCAN: the outputs q_j, j < min(cap, len), of
+// tile t with the top carry C_(t+1) = carry[t + 1] (zero when carry is null: a single
+// tile).
+template <bool SCAN>
+__global__ void __launch_bounds__(kKateT) kate_tile_kernel(const Fr* __restrict__ src, Fr* __restrict__ q,
+    uint64_t len, uint64_t cap, KatePw* __restrict__ pw, uint32_t* __restrict__ scan, Fr* __restrict__ agg,
+    const Fr* __restrict__ carry, uint64_t tiles, unsigned log_g)
+{
+    // the tile's words: 8 planes of kKateTile slots (the scan pass then takes the
+    // first 2 * 9 * kKateT words for the scan)
+    __shared__ uint32_t lds[8 * kKateTile];
+    if (SCAN && blockIdx.x == tiles) {
+        if (threadIdx.x < 64) {
+            Fr29 x = per_lane(pw_p8(pw, kKateT)); // u^kKateTile
+#pragma unroll 1
+            for (unsigned i = 0; i < log_g; ++i)
+                x = Fr29::sqr(x);
+#pragma unroll 1
+            for (unsigned s = 0; s < kKateScanSteps; ++s) {
+                if (threadIdx.x == 0)
+#pragma unroll
+                    for (int w = 0; w < 9; ++w)
+                        pw->carry[s][w] = x.l[w];
+                if (s + 1 < kKateScanSteps)
+                    x = Fr29::sqr(x);
+            }
+        }
+        return; // the whole work-group
+    }
+    const unsigned k = threadIdx.x;
+    const uint64_t base = (uint64_t)blockIdx.x * kKateTile;
+#pragma unroll
+    for (unsigned r = 0; r < kKateE; ++r) {
+        const unsigned e = k + r * kKateT;
+        const uint64_t j = base + e;
+        uint4 a = make_uint4(0, 0, 0, 0), b = make_uint4(0, 0, 0, 0);
+        if (j < len) {
+            const uint4* p = (const uint4*)(src + j + 1);
+            a = p[0];
+            b = p[1];
+        }
+        const unsigned sl = kate_slot(e);
+        lds[0 * kKateTile + sl] = a.x;
+        lds[1 * kKateTile + sl] = a.y;
+        lds[2 * kKateTile + sl] = a.z;
+        lds[3 * kKateTile + sl] = a.w;
+        lds[4 * kKateTile + sl] = b.x;
+        lds[5 * kKateTile + sl] = b.y;
+        lds[6 * kKateTile + sl] = b.z;
+        lds[7 * kKateTile + sl] = b.w;
+    }
+    const Fr29 u = pw_u(pw);
+    auto tile_word = [&](unsigned i) {
+        const unsigned sl = kate_slot(kKateE * k + i);
+        uint32_t v[8];
+#pragma unroll
+        for (int w = 0; w < 8; ++w)
+            v[w] = lds[w * kKateTile + sl];
+        return Fr29::from32(v);
+    };
+    uint32_t* my_scan = scan + blockIdx.x * kKateScanWords;
+    if (SCAN) {
+        __syncthreads();
+        // the thread's bottom value with a zero boundary at its top; limbs < 2^30,
+        // values < 2.02m (a canonical b plus a product below 1.02m)
+        Fr29 x = tile_word(kKateE - 1);
+#pragma unroll
+        for (int i = kKateE - 2; i >= 0; --i)
+            x = Fr29::add(tile_word(i), Fr29::mul(x, u));
+        __syncthreads(); // every thread has its words: the LDS is free
+        const Fr29 s = kate_block_scan(Fr29::reduce_2m(Fr29::normalize(x)), lds,
+            [&](unsigned st) { return pw_p8(pw, 1u << st); });
+#pragma unroll
+        for (int w = 0; w < 9; ++w)
+            my_scan[w * kKateT + k] = s.l[w];
+        if (k == 0)
+            Fr29::reduce_full(s).store32(agg + blockIdx.x);
+        return;
+    }
+    // c_k = S'_(k+1) + u^(8 (kKateT - 1 - k)) C_(t+1): the value just above thread k's
+    // outputs (S'_kKateT = 0)
+    Fr29 c = Fr29::zero();
+    if (k + 1 < kKateT) {
+#pragma unroll
+        for (int w = 0; w < 9; ++w)
+            c.l[w] = my_scan[w * kKateT + k + 1];
+    }
+    if (carry)
+        c = kate_step(c, Fr29::load32(carry + blockIdx.x + 1), pw_p8(pw, kKateT - 1 - k));
+    __syncthreads();
+    // the outputs from the thread's top; each slot is read and rewritten by its thread
+#pragma unroll
+    for (int i = kKateE - 1; i >= 0; --i) {
+        c = Fr29::add(tile_word(i), Fr29::mul(c, u));
+        uint32_t v[8];
+        Fr29::reduce_full(Fr29::normalize(c)).to32(v);
+        const unsigned sl = kate_slot(kKateE * k + i);
+#pragma unroll
+        for (int w = 0; w < 8; ++w)
+            lds[w * kKateTile + sl] = v[w];
+    }
+    __syncthreads();
+    const uint64_t lim = cap < len ? cap : len;
+#pragma unroll
+    for (unsigned r = 0; r < kKateE; ++r) {
+        const unsigned e = k + r * kKateT;
+        const uint64_t j = base + e;
+        if (j < lim) {
+            const unsigned sl = kate_slot(e);
+            uint4* p = (uint4*)(q + j);
+            p[0] = make_uint4(lds[0 * kKateTile + sl], lds[1 * kKateTile + sl], lds[2 * kKateTile + sl],
+                lds[3 * kKateTile + sl]);
+            p[1] = make_uint4(lds[4 * kKateTile + sl], lds[5 * kKateTile + sl], lds[6 * kKateTile + sl],
+                lds[7 * kKateTile + sl]);
+        }
+    }
+}
+
+// One work-group: thread k takes tiles [k g, k g + g) (g = 2^log_g, kKateT g >= tiles),
+// solves them from the top with a zero boundary (C_t = A_t + u^kKateTile C_(t+1)), scans
+// the threads' bottom values (multiplier u^(kKateTile g d) at distance d) and solves its
+// tiles again from its true top carry: carry[t] = C_t for t < tiles, carry[tiles] = 0.
+__global__ void __launch_bounds__(kKateT) kate_carries_kernel(const Fr* __restrict__ agg, Fr* __restrict__ carry,
+    uint64_t tiles, unsigned log_g, const KatePw* __restrict__ pw)
+{
+    __shared__ uint32_t lds[3 * 9 * kKateT];
+    const unsigned k = threadIdx.x;
+    const uint64_t g = 1ull << log_g;
+    const uint64_t lo = (uint64_t)k * g;
+    const Fr29 ut = pw_p8(pw, kKateT);
+    Fr29 x = Fr29::zero();
+#pragma unroll 1
+    for (uint64_t i = g; i-- > 0;)
+        if (lo + i < tiles)
+            x = kate_step(Fr29::load32(agg + lo + i), x, ut);
+    const Fr29 s = kate_block_scan(x, lds, [&](unsigned st) {
+        Fr29 r;
+#pragma unroll
+        for (int w = 0; w < 9; ++w)
+            r.l[w] = pw->carry[st][w];
+        return r;
+    });
+    uint32_t* xc = lds + 2 * 9 * kKateT;
+#pragma unroll
+    for (int w = 0; w < 9; ++w)
+        xc[w * kKateT + k] = s.l[w];
+    __syncthreads();
+    Fr29 c = Fr29::zero();
+    if (k + 1 < kKateT) {
+#pragma unroll
+        for (int w = 0; w < 9; ++w)
+            c.l[w] = xc[w * kKateT + k + 1];
+    }
+#pragma unroll 1
+    for (uint64_t i = g; i-- > 0;)
+        if (lo + i < tiles) {
+            c = kate_step(Fr29::load32(agg + lo + i), c, ut);
+            Fr29::reduce_full(c).store32(carry + lo + i);
+        }
+    if (k == 0)
+        Fr29::zero().store32(carry + tiles);
+}
+
+uint64_t kate_tiles(uint64_t len) { return (len + kKateTile - 1) / kKateTile; }
+
+struct KateBufs {
+    KatePw* pw;
+    uint32_t* scan;
+    Fr* agg;
+    Fr* carry;
+};
+
+template <class WS>
+KateBufs kate_carve(WS& ws, uint64_t len)
+{
+    const uint64_t m = kate_tiles(len);
+    KateBufs b;
+    b.pw = (KatePw*)ws.take(sizeof(KatePw));
+    b.scan = (uint32_t*)ws.take(4 * kKateScanWords * m);
+    b.agg = (Fr*)ws.take(32 * m);
+    b.carry = m > 1 ? (Fr*)ws.take(32 * (m + 1)) : nullptr;
+    return b;
 }
 
 uint64_t kate_scratch(uint64_t n)
 {
     if (n < 2)
         return 0;
-    KateLevel lv[kMaxLevels];
-    Fr *pw[kMaxLevels], *heads[kMaxLevels], *solved[kMaxLevels];
-    const int levels = kate_levels(n - 1, lv);
     Sizer z;
-    kate_carve(z, lv, levels, pw, heads, solved);
+    kate_carve(z, n - 1);
     return z.s.bytes + 256;
 }
 
-// pw[i] = u^i, i <= kRun
-__global__ void kate_powers_kernel(const Fr* __restrict__ d_u, Fr* __restrict__ pw)
-{
-    for (uint32_t i = threadIdx.x; i <= kRun; i += blockDim.x)
-        Fr::pow_u64(Fr::load(d_u), i).store(pw + i);
-}
-
-// Local solve of q_j = src(j + 1) + u q_{j + 1} on run r = [r kRun, e) with a
-// zero boundary q_e = 0: writes loc_j for j < cap into q and the run head
-// loc_{r kRun} into heads[r]. src(i) is read for i in 1..len.
-// The recurrence runs on the 29-bit form (field29.h): u enters in the 2^261 scale
-// (a 5-bit shift of its stored word), so mul(loc, u) is the stored word of loc * u;
-// loc stays below 2.01m (a normalised product below 1.01m plus a canonical word).
-__global__ void kate_local_kernel(const Fr* __restrict__ src, Fr* __restrict__ q, uint64_t len, uint64_t cap,
-    uint64_t runs, const Fr* __restrict__ pw, Fr* __restrict__ heads)
-{
-    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= runs)
-        return;
-    const Fr29 u = Fr29::shl5(Fr29::load32(pw + 1));
-    const uint64_t lo = r * kRun;
-    const uint64_t e = lo + kRun < len ? lo + kRun : len;
-    Fr29 loc = Fr29::load32(src + e);
-    if (e - 1 < cap)
-        Fr29::reduce_full(loc).store32(q + e - 1);
-    for (uint64_t j = e - 1; j-- > lo;) {
-        loc = Fr29::add(Fr29::load32(src + j + 1), Fr29::mul(loc, u));
-        if (j < cap)
-            Fr29::reduce_full(Fr29::normalize(loc)).store32(q + j);
-    }
-    Fr29::reduce_full(Fr29::normalize(loc)).store32(heads + r);
-}
-
-// With Q_s the true value at the head of run s: q_j += u^(e - j) Q_{r + 1},
-// e = (r + 1) kRun, for every j < cap outside the last run.
-__global__ void kate_fix_kernel(Fr* __restrict__ q, uint64_t cap, uint64_t runs, const Fr* __restrict__ pw,
-    const Fr* __restrict__ solved)
-{
-    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= cap)
-        return;
-    const uint64_t r = j / kRun;
-    if (r + 1 >= runs)
-        return;
-    const uint64_t e = (r + 1) * kRun;
-    const Fr29 t = Fr29::mul(Fr29::shl5(Fr29::load32(pw + (e - j))), Fr29::load32(solved + r + 1));
-    Fr29::reduce_full(Fr29::normalize(Fr29::add(Fr29::load32(q + j), t))).store32(q + j);
-}
-
-// q_j = src(j + 1) + u q_{j + 1} for j < len with q_len = 0, written for j < cap.
-// Level i + 1 solves the heads H_r of level i: Q_r = H_r + u^kRun Q_{r + 1}
-// (run r < runs - 1 is full, and Q_runs = 0), i.e. the same recurrence with
-// src'(r + 1) = H_r and u' = u^kRun; the solved heads then fix level i.
+// q_j = src(j + 1) + u q_{j + 1} for j < len with q_len = 0, written for j < cap
+// (src and q must not overlap).
 RustError kate_solve(const Fr* src, Fr* q, uint64_t len, uint64_t cap, const Fr* d_u, Workspace& ws,
     hipStream_t stream)
 {
-    KateLevel lv[kMaxLevels];
-    Fr *pw[kMaxLevels], *heads[kMaxLevels], *solved[kMaxLevels];
-    const int levels = kate_levels(len, lv);
-    kate_carve(ws, lv, levels, pw, heads, solved);
+    const uint64_t m = kate_tiles(len);
+    const KateBufs b = kate_carve(ws, len);
     if (ws.overflow)
         return fail((int)hipErrorInvalidValue, "kate division: workspace too small for %lu outputs",
             (unsigned long)len);
-    const Fr* cur_src = src;
-    Fr* cur_q = q;
-    uint64_t cur_cap = cap;
-    const Fr* cur_u = d_u;
-    for (int i = 0; i < levels; ++i) {
-        kate_powers_kernel<<<1, kT, 0, stream>>>(cur_u, pw[i]);
-        H2_LAUNCH_CHECK();
-        kate_local_kernel<<<blocks_of(lv[i].runs, 64), 64, 0, stream>>>(cur_src, cur_q, lv[i].len, cur_cap,
-            lv[i].runs, pw[i], heads[i] + 1);
-        H2_LAUNCH_CHECK();
-        cur_src = heads[i];
-        cur_q = solved[i];
-        cur_cap = lv[i].runs;
-        cur_u = pw[i] + kRun;
-    }
-    for (int i = levels - 2; i >= 0; --i) {
-        Fr* qi = i == 0 ? q : solved[i - 1];
-        const uint64_t ci = i == 0 ? cap : lv[i].len;
-        if (ci == 0)
-            continue;
-        kate_fix_kernel<<<blocks_of(ci, kT), kT, 0, stream>>>(qi, ci, lv[i].runs, pw[i], solved[i]);
+    if (m + 1 > 0x7fffffffull)
+        return fail((int)hipErrorInvalidValue, "kate division: %lu outputs exceed the grid", (unsigned long)len);
+    unsigned log_g = 0;
+    while (((uint64_t)kKateT << log_g) < m)
+        ++log_g;
+    const uint64_t out_tiles = kate_tiles(cap < len ? cap : len);
+    if (out_tiles == 0)
+        return ok();
+    kate_table_kernel<<<1, kKateT, 0, stream>>>(d_u, b.pw);
+    H2_LAUNCH_CHECK();
+    kate_tile_kernel<true><<<(unsigned)(m + (m > 1 ? 1 : 0)), kKateT, 0, stream>>>(src, nullptr, len, cap, b.pw,
+        b.scan, b.agg, nullptr, m, log_g);
+    H2_LAUNCH_CHECK();
+    if (m > 1) {
+        kate_carries_kernel<<<1, kKateT, 0, stream>>>(b.agg, b.carry, m, log_g, b.pw);
         H2_LAUNCH_CHECK();
     }
+    kate_tile_kernel<false><<<(unsigned)out_tiles, kKateT, 0, stream>>>(src, q, len, cap, b.pw, b.scan, nullptr,
+        b.carry, m, log_g);
+    H2_LAUNCH_CHECK();
     return ok();
 }
 

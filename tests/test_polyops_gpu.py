@@ -170,7 +170,9 @@ def test_sub_short_and_scalar_at_zero(lib, co):
     assert (dv.to_host(d) == exp).all()
 
 
-KATE_LENGTHS = [2, 3, 32, 64, 65, 1024, 4097, 1 << 16, 1 << 20]
+# one tile is 2048 outputs: 2049 / 2050 are one tile and a tile plus one output; 2^19 + 2
+# needs 257 tiles (two per thread of the carries pass), 2^22 + 3 needs 2049 (sixteen)
+KATE_LENGTHS = [2, 3, 32, 64, 65, 1024, 2049, 2050, 4097, 1 << 16, (1 << 19) + 2, 1 << 20, (1 << 22) + 3]
 
 
 @pytest.mark.parametrize("n", KATE_LENGTHS)
