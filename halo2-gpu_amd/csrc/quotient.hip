@@ -29,6 +29,16 @@ __device__ __forceinline__ uint64_t rot(uint64_t idx, int64_t r, int64_t size)
 using R29 = Fr29;
 
 __device__ __forceinline__ R29 ld29(const Fr* p) { return R29::load32(p); }
+// a per-call constant, the same in every lane, held in scalar registers: the products
+// take it as their scalar operand, and the vector registers it would occupy go to
+// more resident waves
+__device__ __forceinline__ R29 uniform(R29 x)
+{
+#pragma unroll
+    for (int w = 0; w < 9; ++w)
+        x.l[w] = __builtin_amdgcn_readfirstlane(x.l[w]);
+    return x;
+}
 __device__ __forceinline__ R29 c261(const R29& stored) { return R29::shl5(stored); } // < 32m, normalised
 // v * y + t (v < 4.1m normalised, t < 2.3m normalised): < 4.1m (v -> 3.3m + 0.19 v), normalised
 __device__ __forceinline__ R29 horner(const R29& v, const R29& y32, const R29& t)
@@ -46,7 +56,7 @@ __global__ void __launch_bounds__(kBlock) quotient_lookups_kernel(Fr* __restrict
     const Fr* __restrict__ l0, const Fr* __restrict__ l_last, const Fr* __restrict__ l_active,
     const Fr* __restrict__ d_beta, const Fr* __restrict__ d_gamma, const Fr* __restrict__ d_y, uint64_t n)
 {
-    const R29 beta = ld29(d_beta), gamma = ld29(d_gamma), y32 = c261(ld29(d_y));
+    const R29 beta = uniform(ld29(d_beta)), gamma = uniform(ld29(d_gamma)), y32 = uniform(c261(ld29(d_y)));
     const R29 one = R29::from_fp(Fr::one());
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -102,9 +112,9 @@ __global__ void __launch_bounds__(kBlock) quotient_permutation_kernel(Fr* __rest
         __syncthreads();
     }
     R29 beta_term = R29::from_fp(s_base * pw[threadIdx.x]); // ext_omega omega^row, < 1.2m
-    const R29 wstep32 = c261(R29::from_fp(s_step)), y32 = c261(R29::from_fp(y));
-    const R29 beta32 = c261(R29::from_fp(beta)), gamma29 = R29::from_fp(gamma);
-    const R29 delta32 = c261(R29::from_fp(delta)), dstart32 = c261(R29::from_fp(delta_start));
+    const R29 wstep32 = uniform(c261(R29::from_fp(s_step))), y32 = uniform(c261(R29::from_fp(y)));
+    const R29 beta32 = uniform(c261(R29::from_fp(beta))), gamma29 = uniform(R29::from_fp(gamma));
+    const R29 delta32 = uniform(c261(R29::from_fp(delta))), dstart32 = uniform(c261(R29::from_fp(delta_start)));
     const R29 one = R29::from_fp(Fr::one());
     for (uint64_t i = tid; i < n; i += stride) {
         const uint64_t rn = rot(i, (int64_t)rot_scale, isize);
