@@ -211,6 +211,45 @@ __global__ void multiply_add_kernel(Fr* acc, const Fr* in, const Fr* s,
         (Fr::load(acc + i) + Fr::load(in + i) * k).store(acc + i);
 }
 
+// acc[i] += sum_j s_j p_j[i] for up to kFoldMax polynomials in one pass (p_j zero at and
+// beyond len[j]): each element of acc is read and written once instead of once per
+// polynomial. On the 29-bit form: the scalars go to the 2^261 scale once (shl5 of their
+// stored words, held in scalar registers) and two products share one reduction
+// (mul_sum: stored words < m and scalars < 32m, so each pair adds < 1.4m); the sum of
+// acc and at most kFoldMax / 2 pairs stays below 13m < 2^261 with limbs normalised
+// after every pair.
+constexpr unsigned kFoldMax = 16;
+struct FoldJobs {
+    const Fr* p[kFoldMax];
+    uint64_t len[kFoldMax];
+};
+
+__device__ __forceinline__ Fr29 fold_scalar(const Fr* s)
+{
+    Fr29 x = Fr29::shl5(Fr29::load32(s));
+#pragma unroll
+    for (int w = 0; w < 9; ++w)
+        x.l[w] = __builtin_amdgcn_readfirstlane(x.l[w]);
+    return x;
+}
+
+__global__ void __launch_bounds__(kT) multiply_add_many_kernel(Fr* __restrict__ acc, const FoldJobs jobs,
+    const Fr* __restrict__ scalars, unsigned count, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        Fr29 a = Fr29::load32(acc + i);
+        unsigned j = 0;
+        for (; j + 1 < count; j += 2) {
+            const Fr29 x = i < jobs.len[j] ? Fr29::load32(jobs.p[j] + i) : Fr29::zero();
+            const Fr29 y = i < jobs.len[j + 1] ? Fr29::load32(jobs.p[j + 1] + i) : Fr29::zero();
+            a = Fr29::normalize(Fr29::add(a, Fr29::mul_sum(x, fold_scalar(scalars + j), y, fold_scalar(scalars + j + 1))));
+        }
+        if (j < count && i < jobs.len[j])
+            a = Fr29::normalize(Fr29::add(a, Fr29::mul(Fr29::load32(jobs.p[j] + i), fold_scalar(scalars + j))));
+        Fr29::reduce_full(a).store32(acc + i);
+    }
+}
+
 __global__ void elementwise_mul_kernel(Fr* out, const Fr* a, const Fr* b, uint64_t n)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
@@ -913,6 +952,28 @@ RustError _halo2_poly_multiply_add(void* d_poly_acc, const void* d_poly_in, cons
     multiply_add_kernel<<<ew_blocks(poly_length), kT, 0, (hipStream_t)stream>>>((Fr*)d_poly_acc, (const Fr*)d_poly_in,
         (const Fr*)d_scalar, poly_length);
     H2_LAUNCH_CHECK();
+    return ok();
+}
+
+RustError h2amd_poly_multiply_add_many(void* d_acc, const void* const* d_polys, const uint64_t* lengths,
+    const void* d_scalars, uint32_t count, uint64_t n, void* stream)
+{
+    for (uint32_t j = 0; j < count; ++j)
+        H2_CHECK_ARG(lengths[j] <= n, "h2amd_poly_multiply_add_many: polynomial %u has %lu coefficients, more than %lu",
+            j, (unsigned long)lengths[j], (unsigned long)n);
+    if (n == 0)
+        return ok();
+    for (uint32_t c0 = 0; c0 < count; c0 += kFoldMax) {
+        const uint32_t m = count - c0 < kFoldMax ? count - c0 : kFoldMax;
+        FoldJobs jobs;
+        for (uint32_t j = 0; j < kFoldMax; ++j) {
+            jobs.p[j] = j < m ? (const Fr*)d_polys[c0 + j] : nullptr;
+            jobs.len[j] = j < m ? lengths[c0 + j] : 0;
+        }
+        multiply_add_many_kernel<<<ew_blocks(n), kT, 0, (hipStream_t)stream>>>((Fr*)d_acc, jobs,
+            (const Fr*)d_scalars + c0, m, n);
+        H2_LAUNCH_CHECK();
+    }
     return ok();
 }
 

@@ -132,6 +132,7 @@ SIGNATURES = {
     "_halo2_multiopen_poly_calculation_workspace_size": (U64, [U64, U64]),
     "_halo2_multiopen_poly_calculation": (RustError, [OBJP, OBJP, SZ, SZ, SZ, OBJP, OBJP, OBJP, P, U64, P]),
     "_halo2_poly_multiply_add": (RustError, [P, P, P, SZ, P]),
+    "h2amd_poly_multiply_add_many": (RustError, [P, P, P, P, U32, U64, P]),
     "_halo2_poly_elementwise_multiply": (RustError, [P, P, P, U64, P]),
     "_halo2_poly_fill_one": (RustError, [P, U64, P]),
     "_halo2_poly_sub_short_inplace": (RustError, [P, P, U64, P]),

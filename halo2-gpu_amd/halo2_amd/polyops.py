@@ -200,6 +200,24 @@ def poly_multiply_add_device(d_acc: torch.Tensor, d_in: torch.Tensor, scalar: in
     poly_multiply_add_device_with_d_scalar(d_acc, d_in, _scalar_dev(scalar))
 
 
+def poly_multiply_add_many_device(d_acc: torch.Tensor, polys: list, d_scalars: torch.Tensor) -> None:
+    """d_acc += sum_j d_scalars[j] * polys[j] in one pass per 16 polynomials
+    (h2amd_poly_multiply_add_many); a polynomial shorter than d_acc counts as zero-padded.
+    The SHPLONK folds of prover.rs:183-268 and 331-386, one call per fold instead of one
+    _halo2_poly_multiply_add per polynomial."""
+    n = _len(d_acc)
+    count = len(polys)
+    assert _len(d_scalars) >= count, "fewer scalars than polynomials"
+    if count == 0:
+        return
+    lens = [_len(p) for p in polys]
+    assert max(lens) <= n, "a polynomial longer than the accumulator"
+    ptrs = (ctypes.c_void_p * count)(*[dv.ptr(p) for p in polys])
+    lengths = (ctypes.c_uint64 * count)(*lens)
+    _lib.check(_lib.load().h2amd_poly_multiply_add_many(dv.ptr(d_acc), ptrs, lengths, dv.ptr(d_scalars), count, n,
+                                                         dv.stream()), "h2amd_poly_multiply_add_many")
+
+
 def poly_multiply_add_device_at_lut_offset(d_acc: torch.Tensor, d_in: torch.Tensor, d_lut: torch.Tensor,
                                            lut_offset: int) -> None:
     """polynomial_ops.rs:397-439: the scalar is entry `lut_offset` of a device table."""

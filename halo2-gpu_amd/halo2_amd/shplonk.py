@@ -174,9 +174,7 @@ def create_proof_device(params, queries: list[ProverQuery], y: int, v: int, u,
         st = streams[si % len(streams)]
         with torch.cuda.stream(st):
             d_p_x = _zeros(n)
-            for i, c in enumerate(rs.commitments):
-                d_in = c.poly if c.poly.numel() == 4 * n else torch.cat([c.poly, _zeros(n - c.poly.numel() // 4)])
-                polyops.poly_multiply_add_device_at_lut_offset(d_p_x, d_in, d_y_powers, i)
+            polyops.poly_multiply_add_many_device(d_p_x, [c.poly for c in rs.commitments], d_y_powers)
             r_x = [0] * max(len(c.low_degree_equivalent) for c in rs.commitments)
             for c, py in zip(rs.commitments, powers(y, len(rs.commitments))):
                 for i, coef in enumerate(c.low_degree_equivalent):
@@ -199,8 +197,7 @@ def create_proof_device(params, queries: list[ProverQuery], y: int, v: int, u,
         t.record_stream(main)
     d_v_powers = _dev_table(powers(v, max(1, len(quotient_polys))))
     d_h_x = _zeros(n)
-    for j, q in enumerate(quotient_polys):
-        polyops.poly_multiply_add_device_at_lut_offset(d_h_x, q, d_v_powers, j)
+    polyops.poly_multiply_add_many_device(d_h_x, quotient_polys, d_v_powers)
     h = params.commit_device(d_h_x)
     if callable(u):
         u = u(h) % R_MOD
@@ -217,8 +214,7 @@ def create_proof_device(params, queries: list[ProverQuery], y: int, v: int, u,
         lin.append(d_l_x)
         z_diffs.append(z_i)
     d_l_x = _zeros(n)
-    for j, poly in enumerate(lin):
-        polyops.poly_multiply_add_device_at_lut_offset(d_l_x, poly, d_v_powers, j)
+    polyops.poly_multiply_add_many_device(d_l_x, lin, d_v_powers)
     zt_eval = evaluate_vanishing_polynomial(super_points, u)
     polyops.poly_multiply_add_device(d_l_x, d_h_x, -zt_eval % R_MOD)
     d_h_final = polyops.kate_division_device(d_l_x, u)

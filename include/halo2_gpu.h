@@ -345,6 +345,15 @@ RustError _halo2_multiopen_poly_calculation(const FFITraitObject* poly_in_many, 
 /* culib.rs:266-274 ; polynomial.cu:435-455 — d_poly_acc[i] += *d_scalar * d_poly_in[i] */
 RustError _halo2_poly_multiply_add(void* d_poly_acc, const void* d_poly_in, const void* d_scalar,
     size_t poly_length, void* stream);
+/* No counterpart in the reference (it folds one device polynomial per
+ * _halo2_poly_multiply_add call, src/cuda/polynomial_ops.rs:397-439, as the SHPLONK prover
+ * does for every rotation set, shplonk/prover.rs:183-268): d_acc[i] += sum_j
+ * d_scalars[j] * d_polys[j][i] for i < n, where d_polys[j] (host array of device pointers)
+ * holds lengths[j] <= n coefficients and is zero beyond them; d_scalars is a device table
+ * of count 32-byte Montgomery elements. One pass over d_acc per 16 polynomials;
+ * stream-ordered, no synchronisation. */
+RustError h2amd_poly_multiply_add_many(void* d_acc, const void* const* d_polys, const uint64_t* lengths,
+    const void* d_scalars, uint32_t count, uint64_t n, void* stream);
 /* culib.rs:276-283 ; polynomial.cu:321-341 — d_out[i] = d_a[i] * d_b[i] (aliasing allowed) */
 RustError _halo2_poly_elementwise_multiply(void* d_out, const void* d_a, const void* d_b, uint64_t length,
     void* stream);

@@ -141,6 +141,29 @@ def test_multiply_add_scale_elementwise_fill(lib, co):
     assert (dv.to_host(d_out) == tile(_fr(co, 1))).all()
 
 
+@pytest.mark.parametrize("count", [1, 2, 3, 16, 17, 33])
+def test_multiply_add_many(lib, co, count):
+    """h2amd_poly_multiply_add_many (the SHPLONK folds in one pass per 16 polynomials)
+    against the oracle's acc + sum_j s_j p_j, with polynomials shorter than the
+    accumulator counted as zero-padded (lengths n, n - 1, n / 2, 1, ...)."""
+    from halo2_amd import device as dv
+    from halo2_amd.polyops import poly_multiply_add_many_device
+
+    print(f"multiply_add_many count={count}", flush=True)
+    n = (1 << 14) + 3
+    lens = [[n, n - 1, n // 2, 1, 4097][j % 5] for j in range(count)]
+    acc = rand_fr(co, n, 900 + count)
+    polys = [rand_fr(co, lens[j], 910 + 7 * count + j) for j in range(count)]
+    scal = rand_fr(co, count, 990 + count)
+    exp = acc.copy()
+    for j in range(count):
+        term = co.field_mul(0, polys[j], np.tile(scal[j], (lens[j], 1)))
+        exp[:lens[j]] = co.field_add(0, exp[:lens[j]], term)
+    d_acc = dv.to_device(acc)
+    poly_multiply_add_many_device(d_acc, [dv.to_device(p) for p in polys], dv.to_device(scal))
+    assert (dv.to_host(d_acc) == exp).all()
+
+
 def test_sub_short_and_scalar_at_zero(lib, co):
     from halo2_amd import device as dv
     from halo2_amd.polyops import (poly_sub_scalar_at_zero_device, poly_sub_short_in_place_device,
