@@ -5,8 +5,23 @@ torch tensors (same bits) on the device; points as (n, 8).
 """
 from __future__ import annotations
 
+import threading
+
 import numpy as np
 import torch
+
+_side = threading.local()
+
+
+def side_streams(count: int) -> list:
+    """Up to `count` side streams of this host thread (created once, default priority):
+    for independent launch chains of one proof that are short or latency-bound."""
+    pool = getattr(_side, "pool", None)
+    if pool is None:
+        pool = _side.pool = []
+    while len(pool) < count:
+        pool.append(torch.cuda.Stream())
+    return pool[:count]
 
 
 def device() -> torch.device:

@@ -460,27 +460,49 @@ def create_proof(params: msm.ParamsKZG, pk: ProvingKey, circuit: Circuit, seed: 
     #    set's value on the last usable row, delta^j running across the sets
     beta, gamma = tr.squeeze(), tr.squeeze()
     d_beta, d_gamma, d_delta, d_omega = (_scalar_dev(v) for v in (beta, gamma, DELTA, dom.omega))
-    d_deltaomega = _scalar_dev(1)  # advanced by delta once per column by the product kernel
     ws = int(lib._halo2_permutation_product_device_inputs_workspace_size(n))
-    scratch = dv.empty_bytes(ws)
     groups = {"advice": circuit.advice, "instance": circuit.instance}
     perm_values = [groups[kind][idx] for kind, idx in cs.permutation_columns]
-    z_perm, prefix = [], _scalar_dev(1)
-    for s, c0 in enumerate(range(0, len(perm_values), cs.chunk_len)):
+
+    def perm_ratio(c0: int) -> torch.Tensor:
         cols = perm_values[c0:c0 + cs.chunk_len]
         sig = pk.sigma_values[c0:c0 + cs.chunk_len]
         ratio = dv.empty_u64(4 * n)
         polyops.poly_fill_one_device(ratio)  # the permutation product multiplies into it
+        scratch = dv.empty_bytes(ws)
+        # delta^c0: the set's first column power, advanced by delta once per column
+        d_deltaomega = _scalar_dev(pow(DELTA, c0, R_MOD))
         _lib.check(lib._halo2_permutation_product_device_inputs(
             ctypes.byref(_lib.obj(dv.ptr(ratio))), _lib.obj_array([dv.ptr(x) for x in sig]),
             _lib.obj_array([dv.ptr(x) for x in cols]), dv.ptr(d_beta), dv.ptr(d_gamma), dv.ptr(d_delta),
             dv.ptr(d_omega), dv.ptr(d_deltaomega), n, 0, len(cols), dv.ptr(scratch), ws, dv.stream()),
             "_halo2_permutation_product_device_inputs")
+        return ratio
+
+    # the sets' ratios (products, one batch inversion each: latency-bound) and the lookup
+    # product are independent; only the permutation scans chain, through the prefix. The
+    # first set runs here, the other sets on one side stream, the lookup on another.
+    starts = list(range(0, len(perm_values), cs.chunk_len))
+    side_a, side_b = dv.side_streams(2)
+    side_a.wait_stream(main)
+    side_b.wait_stream(main)
+    with torch.cuda.stream(side_a):
+        later = [perm_ratio(c0) for c0 in starts[1:]]
+    with torch.cuda.stream(side_b):
+        lratio = lookup_product_device(p_in, p_tab, inp, tab, beta, gamma)
+        z_lookup = _scan_into_z(lratio, _scalar_dev(1), n, seed * 7 + 4)
+    z_perm, prefix = [], _scalar_dev(1)
+    for s, c0 in enumerate(starts):
+        if s == 1:
+            main.wait_stream(side_a)
+        ratio = perm_ratio(c0) if s == 0 else later[s - 1]
         z = _scan_into_z(ratio, prefix, n, seed * 7 + 10 + s)
         z_perm.append(z)
         prefix = z.view(-1, 4)[u]
-    lratio = lookup_product_device(p_in, p_tab, inp, tab, beta, gamma)
-    z_lookup = _scan_into_z(lratio, _scalar_dev(1), n, seed * 7 + 4)
+    for t in later:  # made on side_a, scanned here
+        t.record_stream(main)
+    main.wait_stream(side_b)
+    z_lookup.record_stream(main)
     side.wait_stream(main)
     with torch.cuda.stream(side):
         polys = dom.lagrange_to_coeff_many_device_inputs(z_perm + [z_lookup, p_in, p_tab])

@@ -12,7 +12,6 @@ CPU.
 """
 from __future__ import annotations
 
-import threading
 from dataclasses import dataclass
 
 import numpy as np
@@ -45,19 +44,6 @@ class _Commitment:
 class RotationSet:
     points: list[int]
     commitments: list[_Commitment]
-
-
-_side = threading.local()
-
-
-def _side_streams(count: int) -> list:
-    """Up to `count` side streams of this host thread (created once)."""
-    pool = getattr(_side, "pool", None)
-    if pool is None:
-        pool = _side.pool = []
-    while len(pool) < count:
-        pool.append(torch.cuda.Stream())
-    return pool[:count]
 
 
 def powers(x: int, count: int) -> list[int]:
@@ -167,7 +153,7 @@ def create_proof_device(params, queries: list[ProverQuery], y: int, v: int, u,
     # the rotation sets are independent until h is combined: each set's folds and
     # divisions (latency-bound chains of small launches) run on its own stream
     main = torch.cuda.current_stream()
-    streams = [main] + _side_streams(min(len(rotation_sets), 3) - 1)
+    streams = [main] + dv.side_streams(min(len(rotation_sets), 3) - 1)
     for st in streams[1:]:
         st.wait_stream(main)
     for si, rs in enumerate(rotation_sets):  # quotient_contribution, prover.rs:183-268
