@@ -827,7 +827,7 @@ __global__ void __launch_bounds__(R2_THREADS) r2_scatter_kernel(const uint64_t* 
     const uint32_t* __restrict__ bin_start, const uint32_t* __restrict__ chunk_base,
     const uint32_t* __restrict__ h2off, uint32_t* __restrict__ entries, uint32_t* __restrict__ off0)
 {
-    __shared__ uint32_t cnt[1u << MAX_FB]; // histogram, then local cursor
+    __shared__ uint32_t cnt[1u << MAX_FB]; // histogram, then global run starts
     __shared__ uint32_t loc[1u << MAX_FB]; // local exclusive offsets
     __shared__ uint32_t part[R2_THREADS];
     __shared__ uint32_t sval[R2_CHUNK];
@@ -850,15 +850,16 @@ __global__ void __launch_bounds__(R2_THREADS) r2_scatter_kernel(const uint64_t* 
             off0[((uint64_t)b << sp.FB) + low] = h2off[base + (uint64_t)low * kb];
     }
     __syncthreads();
-    // the chunk stays in registers between the histogram and the placement
+    // the chunk stays in registers between the histogram and the placement; the
+    // histogram's atomics also give each entry its rank among its key's entries
     constexpr uint32_t PER_T = R2_CHUNK / R2_THREADS;
     uint64_t kv[PER_T];
+    uint32_t rank[PER_T];
 #pragma unroll
     for (uint32_t q = 0; q < PER_T; ++q) {
         const uint32_t e = start + q * R2_THREADS + threadIdx.x;
         kv[q] = e < end ? kv1[e] : 0;
-        if (e < end)
-            atomicAdd(&cnt[(uint32_t)(kv[q] >> 32)], 1u);
+        rank[q] = e < end ? atomicAdd(&cnt[(uint32_t)(kv[q] >> 32)], 1u) : 0u;
     }
     __syncthreads();
     // exclusive scan of cnt into loc: FBN / R2_THREADS consecutive keys per thread
@@ -883,7 +884,6 @@ __global__ void __launch_bounds__(R2_THREADS) r2_scatter_kernel(const uint64_t* 
         if (k < FBN) {
             loc[k] = run;
             run += cnt[k];
-            cnt[k] = loc[k];
         }
     }
     __syncthreads();
@@ -891,7 +891,7 @@ __global__ void __launch_bounds__(R2_THREADS) r2_scatter_kernel(const uint64_t* 
     for (uint32_t q = 0; q < PER_T; ++q) {
         if (start + q * R2_THREADS + threadIdx.x < end) {
             const uint32_t low = (uint32_t)(kv[q] >> 32);
-            const uint32_t r = atomicAdd(&cnt[low], 1u);
+            const uint32_t r = loc[low] + rank[q];
             sval[r] = (uint32_t)kv[q];
             skey[r] = (uint16_t)low;
         }
