@@ -94,6 +94,9 @@ def evaluate_h_device(domain, gates: GraphEvaluator, fixed_polys, l0, l_last, l_
     nf, na = len(fixed_polys), len(advice_polys)
     value_parts = []
     sigma_polys = list(permutation.sigma_polys) if permutation is not None and permutation.product_polys else []
+    # the lookup identities' beta, gamma, y: one upload for every part
+    d_lsc = dv.to_device(np.stack([mont_limbs(v % R_MOD) for v in (beta, gamma, y)])) if lookups else None
+    d_sc = [d_lsc[4 * i:4 * i + 4] for i in range(3)] if lookups else []
     for part in range(num_parts):
         key = key_parts.get(part) if key_parts is not None else None
         if key is None:
@@ -132,7 +135,9 @@ def evaluate_h_device(domain, gates: GraphEvaluator, fixed_polys, l0, l_last, l_
             zs, rest = rest[:len(permutation.product_polys)], rest[len(permutation.product_polys):]
             groups = {"fixed": fixed, "advice": advice, "instance": instance}
             cols = [groups[kind][idx] for kind, idx in permutation.columns]
-            tz, ts, tc = _ptr_table(zs), _ptr_table(sig), _ptr_table(cols)
+            ptrs = _ptr_table(list(zs) + list(sig) + list(cols))  # one upload for the three tables
+            tz, ts = ptrs[:len(zs)], ptrs[len(zs):len(zs) + len(sig)]
+            tc = ptrs[len(zs) + len(sig):]
             sc = [_host_scalar(v) for v in (beta, gamma, y, DELTA, beta * ZETA, cur, domain.omega)]
             _lib.check(lib._halo2_quotient_permutation(
                 dv.ptr(values), dv.ptr(l0_e), dv.ptr(ll_e), dv.ptr(la_e), dv.ptr(tz), dv.ptr(ts), dv.ptr(tc),
@@ -141,7 +146,6 @@ def evaluate_h_device(domain, gates: GraphEvaluator, fixed_polys, l0, l_last, l_
 
         if lookups:
             table = dv.empty_u64(4 * n)
-            d_sc = [dv.to_device(mont_limbs(v % R_MOD).reshape(1, 4)) for v in (beta, gamma, y)]
             for lk in lookups:
                 compress_expressions_with_runtime_constants_device(lk.graph, theta, beta, gamma, y, n, 1, pool,
                                                                    challenges, table)

@@ -9,6 +9,7 @@ host polynomials are (n, 4) uint64 numpy arrays; scalars are Python ints.
 from __future__ import annotations
 
 import ctypes
+import functools
 
 import numpy as np
 import torch
@@ -283,6 +284,11 @@ def poly_sub_scalar_at_zero_device(d_buf: torch.Tensor, d_scalar: torch.Tensor) 
 
 # ------------------------------------------------------------- kate division
 
+@functools.lru_cache(maxsize=64)
+def _kate_workspace(n: int) -> int:
+    return int(_lib.load()._halo2_kate_division_workspace_size(n))
+
+
 def _kate(d_a: torch.Tensor, d_root: torch.Tensor, out_len: int | None) -> torch.Tensor:
     lib = _lib.load()
     n = _len(d_a)
@@ -296,7 +302,7 @@ def _kate(d_a: torch.Tensor, d_root: torch.Tensor, out_len: int | None) -> torch
         if out_len == 0:
             return dv.empty_u64(0)[:0]
         out = dv.empty_u64(4 * out_len)
-    ws = int(lib._halo2_kate_division_workspace_size(n))
+    ws = _kate_workspace(n)
     scratch = dv.empty_bytes(ws)
     if out_len is None:
         err = lib._halo2_kate_division_device(dv.ptr(d_a), dv.ptr(out), dv.ptr(d_root), n, dv.ptr(scratch), ws,

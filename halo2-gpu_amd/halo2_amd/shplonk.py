@@ -121,12 +121,28 @@ def construct_intermediate_sets(queries: list[ProverQuery],
     return sets, super_points
 
 
-def _dev_scalar(x: int) -> torch.Tensor:
-    return dv.to_device(mont_limbs(x % R_MOD).reshape(1, 4))
-
-
 def _dev_table(vals: list[int]) -> torch.Tensor:
     return dv.to_device(np.stack([mont_limbs(v % R_MOD) for v in vals]))
+
+
+class TableBuilder:
+    """Host scalars gathered into one device table (one staged upload): add() returns a
+    (start, count) slot, view() the slot's words on the device."""
+
+    def __init__(self):
+        self.vals: list[int] = []
+
+    def add(self, vals: list[int]) -> tuple[int, int]:
+        slot = (len(self.vals), len(vals))
+        self.vals += [v % R_MOD for v in vals]
+        return slot
+
+    def upload(self) -> torch.Tensor:
+        return _dev_table(self.vals or [0])
+
+    @staticmethod
+    def view(dev: torch.Tensor, slot: tuple[int, int]) -> torch.Tensor:
+        return dev[4 * slot[0]:4 * (slot[0] + slot[1])]
 
 
 def _zeros(n: int) -> torch.Tensor:
@@ -147,7 +163,23 @@ def create_proof_device(params, queries: list[ProverQuery], y: int, v: int, u,
         for c in rs.commitments:
             c.low_degree_equivalent = lagrange_interpolate(rs.points, c.evals)
     max_c = max([len(rs.commitments) for rs in rotation_sets] + [1])
-    d_y_powers = _dev_table(powers(y, max_c))
+
+    # every scalar the first half needs, in one upload (each upload is a copy launch and
+    # host work between the short launches of the sets): y and v powers, and per set
+    # r(X)'s coefficients and the set's points
+    table = TableBuilder()
+    t_y = table.add(powers(y, max_c))
+    t_v = table.add(powers(v, max(1, len(rotation_sets))))
+    t_rx, t_pt = [], []
+    for rs in rotation_sets:
+        r_x = [0] * max(len(c.low_degree_equivalent) for c in rs.commitments)
+        for c, py in zip(rs.commitments, powers(y, len(rs.commitments))):
+            for i, coef in enumerate(c.low_degree_equivalent):
+                r_x[i] = (r_x[i] + coef * py) % R_MOD
+        t_rx.append(table.add(r_x))
+        t_pt.append(table.add(rs.points))
+    dev = table.upload()
+    d_y_powers, d_v_powers = table.view(dev, t_y), table.view(dev, t_v)
 
     quotient_polys, p_xs = [], []
     # the rotation sets are independent until h is combined: each set's folds and
@@ -161,15 +193,12 @@ def create_proof_device(params, queries: list[ProverQuery], y: int, v: int, u,
         with torch.cuda.stream(st):
             d_p_x = _zeros(n)
             polyops.poly_multiply_add_many_device(d_p_x, [c.poly for c in rs.commitments], d_y_powers)
-            r_x = [0] * max(len(c.low_degree_equivalent) for c in rs.commitments)
-            for c, py in zip(rs.commitments, powers(y, len(rs.commitments))):
-                for i, coef in enumerate(c.low_degree_equivalent):
-                    r_x[i] = (r_x[i] + coef * py) % R_MOD
             d_n_x = dv.empty_u64(4 * n)
-            polyops.poly_sub_short_out_of_place_device(d_n_x, d_p_x, _dev_table(r_x))
+            polyops.poly_sub_short_out_of_place_device(d_n_x, d_p_x, table.view(dev, t_rx[si]))
             d_q = d_n_x
-            for i, point in enumerate(rs.points):
-                d_root = _dev_scalar(point)
+            points = table.view(dev, t_pt[si])
+            for i in range(len(rs.points)):
+                d_root = points[4 * i:4 * i + 4]
                 if i + 1 == len(rs.points):
                     d_q = polyops.kate_division_device_padded_with_d_root(d_q, d_root, n)
                 else:
@@ -181,30 +210,38 @@ def create_proof_device(params, queries: list[ProverQuery], y: int, v: int, u,
         main.wait_stream(st)
     for t in quotient_polys + p_xs:  # possibly made on a side stream, used on this one from here
         t.record_stream(main)
-    d_v_powers = _dev_table(powers(v, max(1, len(quotient_polys))))
     d_h_x = _zeros(n)
     polyops.poly_multiply_add_many_device(d_h_x, quotient_polys, d_v_powers)
     h = params.commit_device(d_h_x)
     if callable(u):
         u = u(h) % R_MOD
 
-    lin, z_diffs = [], []
-    for rs, d_l_x in zip(rotation_sets, p_xs):  # linearisation_contribution, prover.rs:331-386
+    # the second half's scalars (they depend on u), again in one upload
+    z_diffs, r_is = [], []
+    for rs in rotation_sets:  # linearisation_contribution, prover.rs:331-386
         diffs = [p for p in super_points if p not in rs.points]
-        z_i = evaluate_vanishing_polynomial(diffs, u)
+        z_diffs.append(evaluate_vanishing_polynomial(diffs, u))
         r_i = 0
         for c, py in zip(rs.commitments, powers(y, len(rs.commitments))):
             r_i = (r_i + eval_host(c.low_degree_equivalent, u) * py) % R_MOD
-        polyops.poly_sub_scalar_at_zero_device(d_l_x, _dev_scalar(r_i))
-        polyops.poly_scale_device_with_d_s_minus_one(d_l_x, _dev_scalar(z_i - 1))
+        r_is.append(r_i)
+    zt_eval = evaluate_vanishing_polynomial(super_points, u)
+    z0_inv = pow(z_diffs[0], -1, R_MOD)
+    table2 = TableBuilder()
+    t_r = table2.add(r_is)
+    t_z = table2.add([z - 1 for z in z_diffs])
+    t_last = table2.add([-zt_eval, z0_inv - 1, u])
+    dev2 = table2.upload()
+    d_r, d_z, d_last = table2.view(dev2, t_r), table2.view(dev2, t_z), table2.view(dev2, t_last)
+    lin = []
+    for si, d_l_x in enumerate(p_xs):
+        polyops.poly_sub_scalar_at_zero_device(d_l_x, d_r[4 * si:4 * si + 4])
+        polyops.poly_scale_device_with_d_s_minus_one(d_l_x, d_z[4 * si:4 * si + 4])
         lin.append(d_l_x)
-        z_diffs.append(z_i)
     d_l_x = _zeros(n)
     polyops.poly_multiply_add_many_device(d_l_x, lin, d_v_powers)
-    zt_eval = evaluate_vanishing_polynomial(super_points, u)
-    polyops.poly_multiply_add_device(d_l_x, d_h_x, -zt_eval % R_MOD)
-    d_h_final = polyops.kate_division_device(d_l_x, u)
-    z0_inv = pow(z_diffs[0], -1, R_MOD)
-    polyops.poly_scale_device_with_d_s_minus_one(d_h_final, _dev_scalar(z0_inv - 1))
+    polyops.poly_multiply_add_device_with_d_scalar(d_l_x, d_h_x, d_last[0:4])
+    d_h_final = polyops.kate_division_device_with_d_root(d_l_x, d_last[8:12])
+    polyops.poly_scale_device_with_d_s_minus_one(d_h_final, d_last[4:8])
     h_final = params.commit_device(d_h_final)
     return {"h": h, "h_final": h_final, "h_x": d_h_x, "h_x_final": d_h_final, "l_x": d_l_x}
