@@ -379,28 +379,16 @@ struct F29 {
 
     // The same value as pow_m_minus_2 (u^-1 * 2^261 for a = u * 2^261, i.e.
     // 2^522 / A for the integer A = a mod m; zero maps to zero) by the binary
-    // extended Euclidean algorithm and one product with 2^783. Input normalised,
+    // extended GCD (inv_mod_plain) and one product with 2^783. Input normalised,
     // < 2m; output a normalised representative < 1.01m.
     __device__ static F29 inverse_vt(const F29 a)
     {
         const F29 c = reduce_full(a);
         if (c.is_zero())
             return zero();
-        uint32_t w[8];
+        uint32_t w[8], rw[8];
         c.to32(w);
-        uint64_t x[4], m[4], r[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            x[i] = ((uint64_t)w[2 * i + 1] << 32) | w[2 * i];
-            m[i] = ((uint64_t)P::Base::M[2 * i + 1] << 32) | P::Base::M[2 * i];
-        }
-        inv_mod_binary(x, m, r);
-        uint32_t rw[8];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            rw[2 * i] = (uint32_t)r[i];
-            rw[2 * i + 1] = (uint32_t)(r[i] >> 32);
-        }
+        inv_mod_plain<typename P::Base>(w, rw);
         return mul(from32(rw), from_limbs(P::C783));
     }
 
