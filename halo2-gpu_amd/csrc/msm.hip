@@ -51,8 +51,8 @@ constexpr uint32_t MAX_FB = 11; // low bucket bits sorted in round 2 (LDS histog
 constexpr uint32_t R1_THREADS = 1024;
 constexpr uint32_t R1_PER = 4;
 constexpr uint32_t R1_TILE = R1_THREADS * R1_PER; // points per round-1 tile
-constexpr uint32_t R2_THREADS = 512;
-constexpr uint32_t R2_CHUNK = 8192; // entries per round-2 chunk (staged in LDS)
+constexpr uint32_t R2_THREADS = 1024;
+constexpr uint32_t R2_CHUNK = 16384; // entries per round-2 chunk (staged in LDS)
 constexpr uint32_t LIGHT = 32; // pieces of a split bucket summed by one thread
 constexpr uint32_t HEAVY_THREADS = 256;
 constexpr uint32_t HEAVY_BLOCKS = 512;
