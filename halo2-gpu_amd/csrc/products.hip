@@ -617,19 +617,6 @@ unsigned ew_grid(uint64_t n) { return grid_for(n, kEwBlock, 256u * 16u); }
 // mul(shl5(a), b) (one operand moved to the 2^261 scale by a 5-bit shift, which is
 // cheaper than a product); sums of two canonical words (< 2m, limbs < 2^30) enter a
 // product as they are; results are reduced to canonical words before they are stored.
-// den[i] *= beta * perm[i] + gamma + value[i]        (permutation_commit.h:12-38)
-__global__ void perm_denominator_kernel(Fr* __restrict__ den, const Fr* __restrict__ perm, const Fr* __restrict__ val,
-    const Fr* __restrict__ d_beta, const Fr* __restrict__ d_gamma, uint64_t n)
-{
-    const R29 beta32 = R29::shl5(ld29(d_beta)), gamma = ld29(d_gamma);
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        // beta * perm (normalised, < 1.01m) + gamma + value: < 3.01m, limbs < 2^31
-        const R29 t = R29::add(R29::add(R29::mul(beta32, ld29(perm + i)), gamma), ld29(val + i));
-        st29(R29::mul(R29::shl5(ld29(den + i)), t), den + i);
-    }
-}
-
 // omega^i = lo[i mod 2^kSplit] * hi[i >> kSplit]: the two tables (2^kSplit and
 // ceil(n / 2^kSplit) entries) stay in L2, so no thread raises omega to a power
 constexpr uint32_t kSplit = 11;
@@ -649,37 +636,11 @@ __global__ void split_powers_kernel(Fr* __restrict__ lo, Fr* __restrict__ hi, co
         Fr::pow_u64(w, j << kSplit).store(hi + j);
 }
 
-// num[i] (*)= deltaomega * omega^i * beta + gamma + value[i]   (permutation_commit.h:40-75)
-// `first` initialises num instead of multiplying into it.
-__global__ void perm_numerator_kernel(Fr* __restrict__ num, const Fr* __restrict__ val, const Fr* __restrict__ d_beta,
-    const Fr* __restrict__ d_gamma, const Fr* __restrict__ lo, const Fr* __restrict__ hi,
-    const Fr* __restrict__ d_deltaomega, uint64_t n, int first)
-{
-    const R29 gamma = ld29(d_gamma);
-    // (deltaomega * beta) in the 2^261 scale
-    const R29 c32 = R29::shl5(R29::mul(R29::shl5(ld29(d_deltaomega)), ld29(d_beta)));
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const R29 w = R29::mul(R29::shl5(ld29(lo + (i & ((1u << kSplit) - 1)))), ld29(hi + (i >> kSplit)));
-        const R29 t = R29::add(R29::add(R29::mul(c32, w), gamma), ld29(val + i)); // < 3.01m
-        if (first)
-            st29(t, num + i);
-        else
-            st29(R29::mul(R29::shl5(ld29(num + i)), t), num + i);
-    }
-}
-
 __global__ void mul_into_kernel(Fr* __restrict__ a, const Fr* __restrict__ b, uint64_t n)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
         st29(R29::mul(R29::shl5(ld29(a + i)), ld29(b + i)), a + i);
-}
-
-__global__ void scalar_mul_kernel(Fr* __restrict__ a, const Fr* __restrict__ b)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0)
-        (Fr::load(a) * Fr::load(b)).store(a);
 }
 
 // (A' + beta)(S' + gamma)                             (commit_product.h:12-38)
@@ -707,6 +668,69 @@ __global__ void lookup_numerator_kernel(Fr* __restrict__ out, const Fr* __restri
     }
 }
 
+// Up to kPermCols columns in one pass (permutation_commit.h:12-75, which runs one
+// denominator and one numerator kernel per column): per element, for each column j in
+// order, den *= beta perm_j + gamma + value_j and num (*)= deltaomega_j omega^i beta +
+// gamma + value_j with deltaomega_j = deltaomega delta^j, every product reduced to its
+// canonical word as a per-column pass would store it; den and num are read and written
+// once per launch instead of once per column, and omega^i is formed once.
+constexpr unsigned kPermCols = 4;
+struct PermCols {
+    const Fr* perm[kPermCols];
+    const Fr* val[kPermCols];
+};
+
+__global__ void __launch_bounds__(kEwBlock) perm_columns_kernel(Fr* __restrict__ den, Fr* __restrict__ num,
+    const PermCols cols, unsigned ncols, const Fr* __restrict__ d_beta, const Fr* __restrict__ d_gamma,
+    const Fr* __restrict__ d_delta, const Fr* __restrict__ lo, const Fr* __restrict__ hi,
+    const Fr* __restrict__ d_deltaomega, uint64_t n, int first)
+{
+    const R29 beta = ld29(d_beta), beta32 = R29::shl5(beta), gamma = ld29(d_gamma);
+    // (deltaomega_j beta) in the 2^261 scale; deltaomega_(j+1) the canonical word of
+    // deltaomega_j delta (what the per-column path's scalar product stores)
+    R29 c32[kPermCols];
+    {
+        const R29 delta32 = R29::shl5(ld29(d_delta));
+        R29 dw = ld29(d_deltaomega);
+#pragma unroll
+        for (unsigned j = 0; j < kPermCols; ++j) {
+            c32[j] = R29::shl5(R29::mul(R29::shl5(dw), beta));
+            dw = R29::reduce_full(R29::mul(dw, delta32));
+        }
+    }
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const R29 w = R29::mul(R29::shl5(ld29(lo + (i & ((1u << kSplit) - 1)))), ld29(hi + (i >> kSplit)));
+        R29 d = ld29(den + i);
+        R29 nm = first ? R29::zero() : ld29(num + i);
+#pragma unroll
+        for (unsigned j = 0; j < kPermCols; ++j) {
+            if (j < ncols) {
+                const R29 v = ld29(cols.val[j] + i);
+                // beta * perm (normalised, < 1.01m) + gamma + value: < 3.01m, limbs < 2^31
+                const R29 td = R29::add(R29::add(R29::mul(beta32, ld29(cols.perm[j] + i)), gamma), v);
+                d = R29::reduce_full(R29::normalize(R29::mul(R29::shl5(d), td)));
+                const R29 tn = R29::add(R29::add(R29::mul(c32[j], w), gamma), v); // < 3.01m
+                nm = R29::reduce_full(R29::normalize(first && j == 0 ? tn : R29::mul(R29::shl5(nm), tn)));
+            }
+        }
+        d.store32(den + i);
+        nm.store32(num + i);
+    }
+}
+
+// deltaomega *= delta, k times (the per-column scalar product, in one launch)
+__global__ void advance_kernel(Fr* __restrict__ a, const Fr* __restrict__ b, unsigned k)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        Fr x = Fr::load(a);
+        const Fr y = Fr::load(b);
+        for (unsigned i = 0; i < k; ++i)
+            x = x * y;
+        x.store(a);
+    }
+}
+
 // Folds `batch` columns into the denominator and numerator accumulators; the
 // first column of a call with `init_num` initialises the numerator.
 // `lo`/`hi` hold the split omega powers (split_powers_kernel) for this call.
@@ -714,15 +738,17 @@ RustError permutation_product_columns(Fr* den, Fr* num, const FFITraitObject* pe
     const Fr* d_beta, const Fr* d_gamma, const Fr* d_delta, const Fr* lo, const Fr* hi, Fr* d_deltaomega, uint64_t n,
     uint64_t offset, uint64_t batch, bool init_num, hipStream_t stream)
 {
-    for (uint64_t c = 0; c < batch; ++c) {
-        const Fr* perm = (const Fr*)perms[c].ptr + offset;
-        const Fr* val = (const Fr*)values[c].ptr + offset;
-        perm_denominator_kernel<<<ew_grid(n), kEwBlock, 0, stream>>>(den, perm, val, d_beta, d_gamma, n);
+    for (uint64_t c0 = 0; c0 < batch; c0 += kPermCols) {
+        const unsigned m = batch - c0 < kPermCols ? (unsigned)(batch - c0) : kPermCols;
+        PermCols cols;
+        for (unsigned j = 0; j < kPermCols; ++j) {
+            cols.perm[j] = j < m ? (const Fr*)perms[c0 + j].ptr + offset : nullptr;
+            cols.val[j] = j < m ? (const Fr*)values[c0 + j].ptr + offset : nullptr;
+        }
+        perm_columns_kernel<<<ew_grid(n), kEwBlock, 0, stream>>>(den, num, cols, m, d_beta, d_gamma, d_delta, lo, hi,
+            d_deltaomega, n, init_num && c0 == 0);
         H2_LAUNCH_CHECK();
-        perm_numerator_kernel<<<ew_grid(n), kEwBlock, 0, stream>>>(num, val, d_beta, d_gamma, lo, hi, d_deltaomega,
-            n, init_num && c == 0);
-        H2_LAUNCH_CHECK();
-        scalar_mul_kernel<<<1, 64, 0, stream>>>(d_deltaomega, d_delta);
+        advance_kernel<<<1, 64, 0, stream>>>(d_deltaomega, d_delta, m);
         H2_LAUNCH_CHECK();
     }
     return ok();

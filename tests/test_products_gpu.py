@@ -106,7 +106,9 @@ def _perm_inputs(co, n, batch, seed):
     return cols, perms, sc
 
 
-@pytest.mark.parametrize("n,batch", [(1000, 1), (1 << 14, 3), ((1 << 18) + 1, 4)])
+# up to 4 columns go through one launch: 5 and 9 columns take two and three, the delta
+# power carried between them
+@pytest.mark.parametrize("n,batch", [(1000, 1), (1 << 14, 3), ((1 << 18) + 1, 4), (4099, 5), (1 << 12, 9)])
 def test_permutation_product_device(lib, co, n, batch):
     from halo2_amd import _lib, device as dv
 
