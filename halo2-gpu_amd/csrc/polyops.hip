@@ -303,7 +303,7 @@ unsigned ew_blocks(uint64_t n)
 //    into the true values C_t = q_(tT);
 //  * output pass: thread k's carry c_k = S'_(k+1) + u^(8 (255 - k)) C_(t+1) (one
 //    product, the power from a table), then its 8 outputs by the recurrence itself.
-// 7 + 8 + 1 + 8 products per thread of 8 outputs, at most 16 of them dependent in one
+// 7 + 8 + 1 + 8 products per thread of 8 outputs, at most 15 of them dependent in one
 // pass. Launches: powers of u (one work-group), the scan pass (plus one work-group
 // squaring out the carries' multipliers beside the tiles), the carries (more than one
 // tile only), the outputs.
